@@ -1,0 +1,36 @@
+"""ctypes signatures of every function include/acc_cn.h declares."""
+from __future__ import annotations
+
+import ctypes as C
+
+i32p = C.POINTER(C.c_int32)
+u32p = C.POINTER(C.c_uint32)
+u64p = C.POINTER(C.c_uint64)
+f32p = C.POINTER(C.c_float)
+vp = C.c_void_p
+
+# name -> (restype, argtypes); kept in step with include/acc_cn.h
+SIGNATURES = {
+    "acc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "acc_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "acc_ctx_destroy": (None, [vp]),
+    "acc_ctx_synchronize": (C.c_int, [vp]),
+    "acc_ctx_stream": (vp, [vp]),
+    "acc_last_error": (C.c_char_p, []),
+    "acc_graph_upload": (C.c_int, [vp, C.c_uint32, C.c_uint64, u32p, i32p, C.c_uint32, C.POINTER(vp)]),
+    "acc_graph_release": (None, [vp]),
+    "acc_graph_info": (C.c_int, [vp, u32p, u64p, u64p]),
+    "acc_cn_compute": (C.c_int, [vp, C.c_int, C.c_uint32]),
+    "acc_cn_download": (C.c_int, [vp, i32p, u64p]),
+    "acc_cn_all_edges": (C.c_int, [vp, C.c_int, C.c_uint32, i32p, u64p]),
+    "acc_cn_device_counts": (vp, [vp]),
+    "acc_cn_timings": (C.c_int, [vp, f32p, C.c_int, C.c_char_p, C.c_int]),
+    "acc_triangle_total": (C.c_int, [vp, C.c_int, u64p]),
+}
+
+
+def declare(lib) -> None:
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args

@@ -1,0 +1,165 @@
+"""Python host API over the C ABI of libacc_cn.so.
+
+Mirrors the reference's GPU plugin boundary: ``SCAN_XP::CheckCore(Graph*)``
+(scan-query-cuda/experimental_scan_xp_multigpu_multipass.cu:3-166) leaves
+``common_node_num[i] = 2 + |N(u) ∩ N(v)|`` for every slot, and
+``SCAN_XP::TriCnt()`` (scan-query/scan_xp_triangle_cnt.cpp:62-116) reports the
+triangle total.  Here a :class:`Context` owns a device stream, a
+:class:`ResidentGraph` owns a CSR in HBM, and ``count_all_edges`` /
+``triangle_total`` return the same values.  Errors raise :class:`AccError`
+instead of exiting the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import AccError, check_cn, cn_lib
+
+PATH_AUTO = 0
+PATH_FORWARD = 1
+PATH_BITMAP = 2
+PATH_MERGE = 4
+PATHS = {"auto": PATH_AUTO, "forward": PATH_FORWARD, "bitmap": PATH_BITMAP, "merge": PATH_MERGE}
+
+CN_RAW = 0x1
+CN_TIMING = 0x2
+UPLOAD_NO_VALIDATE = 0x1
+
+
+def _path(p) -> int:
+    return PATHS[p] if isinstance(p, str) else int(p)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    check_cn(cn_lib().acc_device_count(C.byref(c)))
+    return c.value
+
+
+class Context:
+    """One device and one HIP stream (``acc_ctx``)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = cn_lib()
+        h = C.c_void_p()
+        check_cn(self._lib.acc_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        check_cn(self._lib.acc_ctx_synchronize(self._h))
+
+    def stream(self) -> int:
+        return self._lib.acc_ctx_stream(self._h) or 0
+
+    def upload(self, graph, validate: bool = True) -> "ResidentGraph":
+        return ResidentGraph(self, graph, validate)
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.acc_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ResidentGraph:
+    """A CSR resident in HBM (``acc_graph``) plus its count array."""
+
+    def __init__(self, ctx: Context, graph, validate: bool = True):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.n = graph.n
+        self.m = graph.m
+        off = np.ascontiguousarray(graph.node_off, dtype=np.uint32)
+        adj = np.ascontiguousarray(graph.edge_dst, dtype=np.int32)
+        if off.shape[0] != self.n + 1:
+            raise ValueError("node_off must have n+1 entries")
+        h = C.c_void_p()
+        flags = 0 if validate else UPLOAD_NO_VALIDATE
+        check_cn(self._lib.acc_graph_upload(ctx.handle, self.n, self.m, _ffi.ptr(off, C.c_uint32),
+                                            _ffi.ptr(adj, C.c_int32), flags, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self) -> int:
+        n, m, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        check_cn(self._lib.acc_graph_info(self._h, C.byref(n), C.byref(m), C.byref(b)))
+        return b.value
+
+    def compute(self, path="auto", raw: bool = False, timing: bool = False) -> None:
+        flags = (CN_RAW if raw else 0) | (CN_TIMING if timing else 0)
+        check_cn(self._lib.acc_cn_compute(self._h, _path(path), flags))
+
+    def download(self, counts: bool = True):
+        cnt = np.empty(self.m, dtype=np.int32) if counts else None
+        t = C.c_uint64(0)
+        check_cn(self._lib.acc_cn_download(self._h, _ffi.ptr(cnt, C.c_int32) if counts else None,
+                                           C.byref(t)))
+        return cnt, int(t.value)
+
+    def count_all_edges(self, path="auto", raw: bool = False):
+        """Per-slot counts (int32[m]; 2 + |N(u) ∩ N(v)| unless raw) and the triangle total."""
+        cnt = np.empty(self.m, dtype=np.int32)
+        t = C.c_uint64(0)
+        check_cn(self._lib.acc_cn_all_edges(self._h, _path(path), CN_RAW if raw else 0,
+                                            _ffi.ptr(cnt, C.c_int32), C.byref(t)))
+        return cnt, int(t.value)
+
+    def triangle_total(self, path="auto") -> int:
+        t = C.c_uint64(0)
+        check_cn(self._lib.acc_triangle_total(self._h, _path(path), C.byref(t)))
+        return int(t.value)
+
+    def timings(self) -> dict:
+        ms = (C.c_float * 32)()
+        names = C.create_string_buffer(1024)
+        k = self._lib.acc_cn_timings(self._h, ms, 32, names, 1024)
+        if k < 0:
+            check_cn(k)
+        keys = names.value.decode().split(";") if k else []
+        return {key: float(ms[i]) for i, key in enumerate(keys)}
+
+    def device_counts_ptr(self) -> int:
+        return self._lib.acc_cn_device_counts(self._h) or 0
+
+    def release(self) -> None:
+        if self._h:
+            self._lib.acc_graph_release(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.release()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+__all__ = ["Context", "ResidentGraph", "AccError", "device_count", "PATH_AUTO", "PATH_FORWARD",
+           "PATH_BITMAP", "PATH_MERGE", "PATHS"]

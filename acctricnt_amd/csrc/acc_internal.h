@@ -1,0 +1,80 @@
+// Internal declarations shared by the HIP translation units of libacc_cn.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "acc_cn.h"
+
+// Sets the thread's last error and returns code.
+int acc_set_error(int code, const std::string &msg);
+
+#define ACC_HIP_TRY(expr)                                                                   \
+  do {                                                                                      \
+    hipError_t acc_e_ = (expr);                                                             \
+    if (acc_e_ != hipSuccess)                                                               \
+      return acc_set_error(ACC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(acc_e_)); \
+  } while (0)
+
+#define ACC_TRY(expr)        \
+  do {                       \
+    int acc_rc_ = (expr);    \
+    if (acc_rc_) return acc_rc_; \
+  } while (0)
+
+struct acc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  size_t lds_per_block = 0;
+};
+
+// One forward-path work unit: source b, its upper slots [j0, j1), nlow = |N<(b)|.
+struct FwdUnit {
+  int32_t b;
+  uint32_t j0;
+  uint32_t j1;
+  uint32_t nlow;
+};
+
+struct acc_graph {
+  acc_ctx *ctx = nullptr;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t max_deg = 0;
+  uint32_t max_low = 0;  // max |N<(b)|
+  // resident CSR and outputs
+  uint32_t *d_off = nullptr;
+  int32_t *d_adj = nullptr;
+  int32_t *d_cnt = nullptr;
+  uint32_t *d_rev = nullptr;            // scratch: reverse slot of every upper slot, ~0 elsewhere
+  unsigned long long *d_sum = nullptr;  // sum over upper slots of the raw counts (= 3T)
+  int32_t *d_flags = nullptr;           // validation flags
+  // forward plan
+  FwdUnit *d_fwd_small = nullptr;  // |N<(b)| fits the LDS table
+  uint32_t n_fwd_small = 0;
+  FwdUnit *d_fwd_big = nullptr;  // |N<(b)| beyond the LDS table: global-memory lookups
+  uint32_t n_fwd_big = 0;
+  uint64_t device_bytes = 0;
+  // last results / timing
+  bool have_result = false;
+  std::vector<hipEvent_t> events;
+  std::vector<std::string> event_names;
+  bool timing_valid = false;
+};
+
+// Kernel launchers (k_forward.hip).
+namespace accfwd {
+constexpr int kWG = 256;
+constexpr int kTableBits = 12;                 // LDS hash table of 4096 keys
+constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
+constexpr uint32_t kChunk = 512;               // upper slots per work unit
+int launch_init(acc_graph *g);
+int launch_count(acc_graph *g);
+int launch_combine(acc_graph *g, int plus);
+int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
+                    hipStream_t s);
+}  // namespace accfwd

@@ -1,0 +1,282 @@
+// C ABI of libacc_cn.so (include/acc_cn.h): contexts, resident graphs,
+// work plans and the counting pipeline.
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "acc_internal.h"
+
+namespace {
+thread_local std::string g_err;
+
+struct DeviceGuard {
+  explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+template <typename T>
+int dev_alloc(acc_graph *g, T **p, uint64_t count) {
+  const size_t bytes = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return acc_set_error(ACC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                         hipGetErrorString(e));
+  }
+  g->device_bytes += bytes;
+  return ACC_OK;
+}
+
+void free_graph(acc_graph *g) {
+  if (!g) return;
+  DeviceGuard guard(g->ctx->device);
+  for (void *p : {static_cast<void *>(g->d_off), static_cast<void *>(g->d_adj), static_cast<void *>(g->d_cnt),
+                  static_cast<void *>(g->d_rev), static_cast<void *>(g->d_sum), static_cast<void *>(g->d_flags),
+                  static_cast<void *>(g->d_fwd_small), static_cast<void *>(g->d_fwd_big)})
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
+  delete g;
+}
+
+// Forward plan: for every source b with upper slots (neighbors > b), units of
+// at most kChunk upper slots; sources whose |N<(b)| exceeds the LDS table go
+// to the global-memory variant.
+void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &small,
+                        std::vector<FwdUnit> &big, uint32_t &max_low, uint32_t &max_deg) {
+  max_low = 0;
+  max_deg = 0;
+  for (uint32_t b = 0; b < n; b++) {
+    const int32_t *lb = std::lower_bound(adj + off[b], adj + off[b + 1], static_cast<int32_t>(b));
+    const uint32_t hs = static_cast<uint32_t>(lb - adj);
+    const uint32_t nl = hs - off[b];
+    max_low = std::max(max_low, nl);
+    max_deg = std::max(max_deg, off[b + 1] - off[b]);
+    for (uint32_t j0 = hs; j0 < off[b + 1]; j0 += accfwd::kChunk) {
+      FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + accfwd::kChunk, off[b + 1]), nl};
+      (nl <= accfwd::kLdsCap ? small : big).push_back(u);
+    }
+  }
+}
+
+int record(acc_graph *g, bool timing, const char *name) {
+  if (!timing) return ACC_OK;
+  hipEvent_t e;
+  ACC_HIP_TRY(hipEventCreate(&e));
+  g->events.push_back(e);
+  g->event_names.emplace_back(name);
+  ACC_HIP_TRY(hipEventRecord(e, g->ctx->stream));
+  return ACC_OK;
+}
+
+}  // namespace
+
+int acc_set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+extern "C" {
+
+const char *acc_last_error(void) { return g_err.c_str(); }
+
+int acc_device_count(int *count) {
+  if (!count) return acc_set_error(ACC_EINVAL, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return ACC_OK;
+}
+
+int acc_ctx_create(int device, acc_ctx **out) {
+  if (!out) return acc_set_error(ACC_EINVAL, "null out");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return acc_set_error(ACC_EDEVICE, "no HIP device is visible");
+  if (device < 0 || device >= count) return acc_set_error(ACC_EINVAL, "device index out of range");
+  ACC_HIP_TRY(hipSetDevice(device));
+  auto *ctx = new (std::nothrow) acc_ctx;
+  if (!ctx) return acc_set_error(ACC_ENOMEM, "host allocation failed");
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return acc_set_error(ACC_EDEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+  }
+  ctx->num_cus = prop.multiProcessorCount;
+  ctx->lds_per_block = prop.sharedMemPerBlock;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return acc_set_error(ACC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return ACC_OK;
+}
+
+void acc_ctx_destroy(acc_ctx *ctx) {
+  if (!ctx) return;
+  DeviceGuard guard(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+int acc_ctx_synchronize(acc_ctx *ctx) {
+  if (!ctx) return acc_set_error(ACC_EINVAL, "null context");
+  DeviceGuard guard(ctx->device);
+  ACC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ACC_OK;
+}
+
+void *acc_ctx_stream(acc_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
+
+int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
+                     uint32_t flags, acc_graph **out) {
+  if (!ctx || !out || !node_off || (m && !edge_dst)) return acc_set_error(ACC_EINVAL, "null argument");
+  *out = nullptr;
+  if (m >= (1ULL << 32)) return acc_set_error(ACC_EINVAL, "m must be below 2^32 (uint32 node_off)");
+  if (n >= (1u << 31)) return acc_set_error(ACC_EINVAL, "n must be below 2^31 (int32 ids)");
+  if (node_off[0] != 0 || node_off[n] != m) return acc_set_error(ACC_EFORMAT, "node_off[0] != 0 or node_off[n] != m");
+  for (uint32_t u = 0; u < n; u++)
+    if (node_off[u + 1] < node_off[u])
+      return acc_set_error(ACC_EFORMAT, "node_off not monotone at vertex " + std::to_string(u));
+  DeviceGuard guard(ctx->device);
+  auto *g = new (std::nothrow) acc_graph;
+  if (!g) return acc_set_error(ACC_ENOMEM, "host allocation failed");
+  g->ctx = ctx;
+  g->n = n;
+  g->m = m;
+  int rc = ACC_OK;
+  auto bail = [&](int code) {
+    free_graph(g);
+    return code;
+  };
+  if ((rc = dev_alloc(g, &g->d_off, static_cast<uint64_t>(n) + 1))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_adj, m + 64))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_cnt, m + 4))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_rev, m + 4))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_sum, 1))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_flags, 4))) return bail(rc);
+  hipError_t e = hipMemcpyAsync(g->d_off, node_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && m)
+    e = hipMemcpyAsync(g->d_adj, edge_dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(g->d_flags, 0, sizeof(int32_t) * 4, ctx->stream);
+  if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
+  if (!(flags & ACC_UPLOAD_NO_VALIDATE)) {
+    if ((rc = accfwd::launch_validate(g->d_off, g->d_adj, n, m, g->d_flags, ctx->stream))) return bail(rc);
+    int32_t hf = 0;
+    e = hipMemcpyAsync(&hf, g->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("validate: ") + hipGetErrorString(e)));
+    if (hf & 1) return bail(acc_set_error(ACC_EFORMAT, "neighbor id out of range"));
+    if (hf & 2) return bail(acc_set_error(ACC_EFORMAT, "Self loop"));
+    if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
+    if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
+  }
+  std::vector<FwdUnit> small, big;
+  build_forward_plan(n, node_off, edge_dst, small, big, g->max_low, g->max_deg);
+  g->n_fwd_small = static_cast<uint32_t>(small.size());
+  g->n_fwd_big = static_cast<uint32_t>(big.size());
+  if (!small.empty()) {
+    if ((rc = dev_alloc(g, &g->d_fwd_small, small.size()))) return bail(rc);
+    e = hipMemcpyAsync(g->d_fwd_small, small.data(), sizeof(FwdUnit) * small.size(), hipMemcpyHostToDevice,
+                       ctx->stream);
+    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
+  }
+  if (!big.empty()) {
+    if ((rc = dev_alloc(g, &g->d_fwd_big, big.size()))) return bail(rc);
+    e = hipMemcpyAsync(g->d_fwd_big, big.data(), sizeof(FwdUnit) * big.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
+  }
+  e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
+  *out = g;
+  return ACC_OK;
+}
+
+void acc_graph_release(acc_graph *g) { free_graph(g); }
+
+int acc_graph_info(const acc_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (n) *n = g->n;
+  if (m) *m = g->m;
+  if (device_bytes) *device_bytes = g->device_bytes;
+  return ACC_OK;
+}
+
+int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (path == ACC_PATH_AUTO) path = ACC_PATH_FORWARD;
+  if (path != ACC_PATH_FORWARD) return acc_set_error(ACC_EINVAL, "unknown counting path " + std::to_string(path));
+  DeviceGuard guard(g->ctx->device);
+  const bool timing = (flags & ACC_CN_TIMING) != 0;
+  for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
+  g->events.clear();
+  g->event_names.clear();
+  g->timing_valid = false;
+  ACC_TRY(record(g, timing, "begin"));
+  ACC_TRY(accfwd::launch_init(g));
+  ACC_TRY(record(g, timing, "k_init"));
+  ACC_TRY(accfwd::launch_count(g));
+  ACC_TRY(record(g, timing, "k_forward_count"));
+  ACC_TRY(accfwd::launch_combine(g, (flags & ACC_CN_RAW) ? 0 : 2));
+  ACC_TRY(record(g, timing, "k_combine"));
+  g->have_result = true;
+  g->timing_valid = timing;
+  return ACC_OK;
+}
+
+int acc_cn_download(acc_graph *g, int32_t *cnt_host, uint64_t *triangles) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (!g->have_result) return acc_set_error(ACC_EINVAL, "no counts computed yet");
+  DeviceGuard guard(g->ctx->device);
+  if (cnt_host && g->m)
+    ACC_HIP_TRY(hipMemcpyAsync(cnt_host, g->d_cnt, sizeof(int32_t) * g->m, hipMemcpyDeviceToHost, g->ctx->stream));
+  unsigned long long s = 0;
+  ACC_HIP_TRY(hipMemcpyAsync(&s, g->d_sum, sizeof(s), hipMemcpyDeviceToHost, g->ctx->stream));
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  if (triangles) *triangles = s / 3;
+  return ACC_OK;
+}
+
+int acc_cn_all_edges(acc_graph *g, int path, uint32_t flags, int32_t *cnt_host, uint64_t *triangles) {
+  ACC_TRY(acc_cn_compute(g, path, flags));
+  return acc_cn_download(g, cnt_host, triangles);
+}
+
+const int32_t *acc_cn_device_counts(const acc_graph *g) { return g ? g->d_cnt : nullptr; }
+
+int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (!g->timing_valid) return 0;
+  DeviceGuard guard(g->ctx->device);
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  std::string joined;
+  int k = 0;
+  for (size_t i = 1; i < g->events.size(); i++, k++) {
+    float t = 0;
+    ACC_HIP_TRY(hipEventElapsedTime(&t, g->events[i - 1], g->events[i]));
+    if (ms && k < cap) ms[k] = t;
+    if (!joined.empty()) joined += ";";
+    joined += g->event_names[i];
+  }
+  if (names && names_cap > 0) {
+    std::strncpy(names, joined.c_str(), static_cast<size_t>(names_cap) - 1);
+    names[names_cap - 1] = '\0';
+  }
+  return k;
+}
+
+int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles) {
+  if (!triangles) return acc_set_error(ACC_EINVAL, "null output");
+  ACC_TRY(acc_cn_compute(g, path, 0));
+  return acc_cn_download(g, nullptr, triangles);
+}
+
+}  // extern "C"

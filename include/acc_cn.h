@@ -1,0 +1,102 @@
+/*
+ * acc_cn.h -- C ABI of libacc_cn.so, the MI355X (gfx950) all-edge
+ * common-neighbor library.
+ *
+ * This is the drop-in boundary for the reference's GPU plugin
+ *   void SCAN_XP::CheckCore(Graph *g)
+ *     (scan-query-cuda/scan_xp.h:36, defined at
+ *      scan-query-cuda/experimental_scan_xp_multigpu_multipass.cu:3-166)
+ * and for the triangle entry
+ *   void SCAN_XP::TriCnt()        (scan-query/scan_xp.h:250,
+ *                                   scan-query/scan_xp_triangle_cnt.cpp:62-116)
+ * The C++ host classes in acctricnt_amd/csrc/host/ keep those signatures and
+ * call the functions below.  Plain pointers and sizes only; no exit() on
+ * error: every function returns 0 (ACC_OK) or a negative ACC_E* code from
+ * acc_graph.h, and acc_last_error() gives the text of the calling thread's
+ * last error.
+ *
+ * Threading: one acc_ctx owns one HIP stream on one device.  Calls on one
+ * context must not overlap; different contexts may be used from different
+ * threads.  All device work of a context runs on its stream.
+ */
+#ifndef ACC_CN_H
+#define ACC_CN_H
+
+#include <stdint.h>
+
+#include "acc_graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct acc_ctx acc_ctx;
+typedef struct acc_graph acc_graph;
+
+/* Counting paths.  All return identical counts. */
+#define ACC_PATH_AUTO 0    /* the tuned choice for the graph (see acc_tune) */
+#define ACC_PATH_FORWARD 1 /* degree-ordered forward lists: per source b an LDS hash of N<(b),
+                              probe N(c) ∩ [0,b) for every c > b; one count per triangle role */
+#define ACC_PATH_BITMAP 2  /* per source u an N(u) membership set (LDS, HBM bitmap for hubs),
+                              probe N(v) for every v > u: the reference BMP kernel's algorithm
+                              (scan-query-cuda/scan_xp_multi_gpu_multi_pass_common.cu:308-407) */
+#define ACC_PATH_MERGE 4   /* merge path over both sorted lists, LDS-staged windows:
+                              the reference MPS kernels' algorithm (same file :32-176) */
+
+/* Flags for acc_cn_compute / acc_cn_all_edges. */
+#define ACC_CN_RAW 0x1u     /* store |N(u) ∩ N(v)| instead of the reference's 2 + |N(u) ∩ N(v)| */
+#define ACC_CN_TIMING 0x2u  /* record per-kernel HIP events; read them with acc_cn_timings */
+
+/* Upload flags. */
+#define ACC_UPLOAD_NO_VALIDATE 0x1u /* skip the host CSR validation (trusted input) */
+
+/* Context -------------------------------------------------------------- */
+int acc_device_count(int *count);
+int acc_ctx_create(int device, acc_ctx **out);
+void acc_ctx_destroy(acc_ctx *ctx);
+int acc_ctx_synchronize(acc_ctx *ctx);
+/* The context's HIP stream (a hipStream_t), for callers that order their own work after it. */
+void *acc_ctx_stream(acc_ctx *ctx);
+const char *acc_last_error(void);
+
+/* Resident graphs ------------------------------------------------------- */
+/* Copies a host CSR (uint32 node_off[n+1], int32 edge_dst[m], the layout of
+ * struct Graph, scan-query-cuda/util/graph.h:22-61) into HBM, validates it as
+ * Graph::CheckInputGraph does (scan-query-cuda/util/graph.cu:196-214) plus
+ * symmetry and id range (ACC_EFORMAT on failure, instead of exit(1)), and
+ * prepares the per-graph work plans. */
+int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_off,
+                     const int32_t *edge_dst, uint32_t flags, acc_graph **out);
+void acc_graph_release(acc_graph *g);
+/* n, m and the device bytes the graph holds (CSR, counts, scratch, plans). */
+int acc_graph_info(const acc_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes);
+
+/* All-edge counting ------------------------------------------------------ */
+/* Enqueue the count of every slot on the context stream.  On completion the
+ * graph's device count array holds cnt[i] = 2 + |N(u) ∩ N(v)| for every slot
+ * i = (u,v) (|N(u) ∩ N(v)| with ACC_CN_RAW), the postcondition of the
+ * reference's CheckCore on common_node_num (scan-query-cuda/scan_xp_common.cu:220-237),
+ * and the triangle total T = sum over u<v slots of |N(u) ∩ N(v)| / 3. */
+int acc_cn_compute(acc_graph *g, int path, uint32_t flags);
+/* Synchronise and copy the counts (int32[m], may be NULL) and T (may be NULL) to the host. */
+int acc_cn_download(acc_graph *g, int32_t *cnt_host, uint64_t *triangles);
+/* compute + download in one call. */
+int acc_cn_all_edges(acc_graph *g, int path, uint32_t flags, int32_t *cnt_host, uint64_t *triangles);
+/* Device address of the graph's int32[m] count array (valid until release). */
+const int32_t *acc_cn_device_counts(const acc_graph *g);
+
+/* Per-kernel times (ms) of the last acc_cn_compute issued with ACC_CN_TIMING:
+ * up to cap entries, names written as a ';'-separated list into names
+ * (names_cap bytes).  Returns the number of kernels. */
+int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap);
+
+/* Triangle count (SCAN_XP::TriCnt / TriCntSortedArray,
+ * scan-query/scan_xp_triangle_cnt.cpp:778-957): runs the counting path and
+ * returns T. */
+int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACC_CN_H */

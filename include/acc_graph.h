@@ -1,0 +1,103 @@
+/*
+ * acc_graph.h -- host-side graph utilities of acctricnt_amd (C ABI, no GPU).
+ *
+ * Builds the sorted, symmetric int32 CSR that the all-edge common-neighbor
+ * path consumes, reads and writes the reference's on-disk format, and
+ * generates the seeded synthetic graphs used by tests and benchmarks.
+ *
+ * CSR contract (same as the reference's struct Graph,
+ * scan-query/util/graph.h:36-77): node_off is uint32[n+1], edge_dst is
+ * int32[m] with every adjacency list strictly increasing, no self-loops,
+ * and (u,v) present iff (v,u) present.  m = number of directed slots.
+ *
+ * Every function returns 0 on success or a negative ACC_E* code; the text of
+ * the last error on the calling thread is available from acc_graph_last_error().
+ */
+#ifndef ACC_GRAPH_H
+#define ACC_GRAPH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct acc_hgraph acc_hgraph; /* opaque host CSR */
+
+/* Error codes shared with acc_cn.h. */
+#define ACC_OK 0
+#define ACC_EINVAL (-1)   /* bad argument */
+#define ACC_EFORMAT (-2)  /* malformed CSR or file: self-loop, unsorted, asymmetric, out of range */
+#define ACC_ENOMEM (-3)   /* host or device allocation failed */
+#define ACC_EIO (-4)      /* file could not be opened, read or written */
+#define ACC_EDEVICE (-5)  /* HIP runtime error, kernel launch failure or missing device */
+#define ACC_EBUDGET (-6)  /* request exceeds a configured memory budget */
+#define ACC_ENOTFOUND (-7)
+
+const char *acc_graph_last_error(void);
+
+/* Accessors.  Pointers stay valid until acc_hgraph_free or a relabel. */
+uint32_t acc_hgraph_n(const acc_hgraph *g);
+uint64_t acc_hgraph_m(const acc_hgraph *g);
+const uint32_t *acc_hgraph_offsets(const acc_hgraph *g);
+const int32_t *acc_hgraph_adj(const acc_hgraph *g);
+void acc_hgraph_free(acc_hgraph *g);
+
+/* Build from an undirected edge list: self-loops dropped, both directions
+ * inserted, duplicates removed, lists sorted.  Ids must lie in [0, n). */
+int acc_hgraph_from_edges(uint32_t n, uint64_t num_edges, const int32_t *src,
+                          const int32_t *dst, acc_hgraph **out);
+
+/* Build from caller CSR arrays (copied), after validating the CSR contract.
+ * Validation follows Graph::CheckInputGraph (scan-query/util/graph.cpp:122-143)
+ * and additionally checks symmetry and id range. */
+int acc_hgraph_from_csr(uint32_t n, uint64_t m, const uint32_t *node_off,
+                        const int32_t *edge_dst, acc_hgraph **out);
+
+/* Validate a CSR without copying it: returns ACC_OK or ACC_EFORMAT. */
+int acc_csr_validate(uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst);
+
+/* Relabel vertices by degree, descending, ties by old id ascending (a stable
+ * version of SCAN_XP::ReorderDegDescending, scan-query/scan_xp.cpp:1279-1301,
+ * whose unstable sort leaves tie order unspecified).  If old_of_new is not
+ * NULL it receives n entries: old_of_new[new_id] = old_id. */
+int acc_hgraph_relabel_degree(acc_hgraph *g, int32_t *old_of_new);
+
+/* Seeded generators.  All randomness is counter-based (SplitMix64 keyed by
+ * seed and sample index), so results do not depend on the thread count. */
+
+/* Erdos-Renyi G(n, M)-style: num_samples uniform pairs (u != v). */
+int acc_gen_er(uint32_t n, uint64_t num_samples, uint64_t seed, acc_hgraph **out);
+
+/* R-MAT with quadrant probabilities a, b, c (d = 1-a-b-c) over 2^scale ids;
+ * samples whose endpoints fall at or beyond n are rejected and redrawn, so
+ * n need not be a power of two.  num_samples accepted samples are drawn. */
+int acc_gen_rmat(uint32_t n, int scale, uint64_t num_samples, double a, double b, double c,
+                 uint64_t seed, acc_hgraph **out);
+
+/* Chung-Lu: expected degree of vertex i proportional to (i+1)^(-1/(gamma-1)),
+ * scaled so the mean expected degree is avg_deg with every expected degree
+ * capped at max_deg; n*avg_deg/2 endpoint pairs drawn by weight. */
+int acc_gen_chung_lu(uint32_t n, double gamma, double avg_deg, double max_deg, uint64_t seed,
+                     acc_hgraph **out);
+
+/* On-disk format of the reference (scan-query/util/graph.cpp:48-120,
+ * scan-query/converter/EdgeListToBinsConverter.cpp:87-126):
+ *   <dir>/b_degree.bin = int32 4, uint32 n, uint32 m, int32 deg[n]
+ *   <dir>/b_adj.bin    = int32 adj[m], lists concatenated in vertex order. */
+int acc_hgraph_write_bins(const acc_hgraph *g, const char *dir);
+int acc_hgraph_read_bins(const char *dir, acc_hgraph **out);
+
+/* Text edge list -> b_degree.bin/b_adj.bin, following
+ * EdgeListToBinsConverter (scan-query/converter/EdgeListToBinsConverter.cpp:24-126):
+ * '#' lines skipped, only lines with first < second kept, n = max id + 1. */
+int acc_convert_edge_list(const char *txt_path, const char *out_dir);
+
+/* Reverse-slot index on the host: rev[i] = slot of (v,u) for slot i = (u,v). */
+int acc_hgraph_reverse_slots(const acc_hgraph *g, uint32_t *rev_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACC_GRAPH_H */

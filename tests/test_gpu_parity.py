@@ -1,0 +1,174 @@
+"""Parity of the HIP counting paths with the oracle, through the C ABI.
+
+Bit-exact comparison of every slot count and the triangle total on seeded
+graphs the oracle finishes in seconds; at the full R-MAT scale-22 size,
+size-independent properties (reverse-slot symmetry, sum of counts = 6T,
+T against the oracle's triangle count) plus bit-exact checks of a sample of
+sources.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import bruteforce as BF
+import oracle
+from acctricnt_amd import graph as G
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+PATHS = ["forward"]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def ctx(built):
+    from acctricnt_amd import cn
+
+    assert cn.device_count() > 0, "no HIP device visible"
+    c = cn.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_counts(ctx, g, path="forward", raw=False):
+    with ctx.upload(g) as rg:
+        return rg.count_all_edges(path, raw=raw)
+
+
+def check_against_oracle(ctx, g, path):
+    print(f"[parity] {g.name} n={g.n} m={g.m} path={path}", flush=True)
+    cnt, t = gpu_counts(ctx, g, path)
+    ref = oracle.cn_merge(g)
+    bad = np.nonzero(cnt != ref)[0]
+    assert bad.size == 0, f"{bad.size} slots differ, first {bad[:5]} gpu {cnt[bad[:5]]} ref {ref[bad[:5]]}"
+    assert t == oracle.triangle_count(g)
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_golden(ctx, name, path):
+    from test_oracle import golden_graph
+
+    ent = GOLDEN[name]
+    g = golden_graph(name)
+    print(f"[golden] {name} path={path}", flush=True)
+    cnt, t = gpu_counts(ctx, g, path)
+    assert sha(cnt) == ent["cnt_sha256"]
+    assert t == ent["triangles"]
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("maker", [
+    lambda: G.gen_er(16384, 16, seed=20190101),
+    lambda: G.gen_rmat(16, 16, seed=1),
+    lambda: G.gen_rmat(18, 16, seed=2),
+    lambda: G.gen_rmat(14, 16, seed=3, relabel=False),
+    lambda: G.gen_chung_lu(1 << 16, 2.1, 32, 1 << 13, seed=4),
+    lambda: G.gen_chung_lu(1 << 15, 2.1, 24, 1 << 14, seed=5, relabel=True),
+], ids=["er16384", "rmat16", "rmat18", "rmat14_unrelabelled", "chunglu16", "chunglu15_relabel"])
+def test_seeded_graphs(ctx, maker, path):
+    check_against_oracle(ctx, maker(), path)
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("seed", range(8))
+def test_random_small(ctx, seed, path):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(1, 300))
+    k = int(rng.integers(0, n * 8 + 1))
+    g = G.from_edges(n, rng.integers(0, n, k), rng.integers(0, n, k), name=f"rand{seed}")
+    check_against_oracle(ctx, g, path)
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_closed_forms(ctx, path):
+    from math import comb
+
+    for k in (2, 3, 5, 16, 70):
+        cnt, t = gpu_counts(ctx, BF.complete(k), path)
+        assert (cnt == k).all() and t == comb(k, 3)
+    cnt, t = gpu_counts(ctx, BF.wheel(9), path)
+    assert t == 9
+    cnt, t = gpu_counts(ctx, BF.windmill(6), path)
+    assert (cnt == 3).all() and t == 6
+    for g in (BF.complete_bipartite(4, 9), BF.cycle(10), BF.star(300)):
+        cnt, t = gpu_counts(ctx, g, path)
+        assert (cnt == 2).all() and t == 0
+
+
+@pytest.mark.parametrize("path", PATHS)
+def test_edge_cases(ctx, path):
+    # one vertex, no edges
+    g = G.from_csr(np.array([0, 0], np.uint32), np.zeros(0, np.int32))
+    cnt, t = gpu_counts(ctx, g, path)
+    assert cnt.size == 0 and t == 0
+    # isolated vertices around a triangle
+    g = G.from_edges(10, [2, 3, 2], [3, 7, 7])
+    cnt, t = gpu_counts(ctx, g, path)
+    assert (cnt == 3).all() and t == 1
+    # a hub whose forward list exceeds the LDS table: K_{1}+clique of 2600 forces |N<(b)| > 2048
+    k = 2600
+    s, d = np.triu_indices(k, 1)
+    g = G.from_edges(k, s, d)
+    cnt, t = gpu_counts(ctx, g, path)
+    assert (cnt == k).all()
+
+
+def test_raw_flag(ctx):
+    g = G.gen_rmat(12, 16, seed=9)
+    c2, t2 = gpu_counts(ctx, g, "forward")
+    c0, t0 = gpu_counts(ctx, g, "forward", raw=True)
+    assert np.array_equal(c2, c0 + 2) and t2 == t0
+
+
+def test_repeat_is_deterministic(ctx):
+    g = G.gen_rmat(15, 16, seed=10)
+    with ctx.upload(g) as rg:
+        a, ta = rg.count_all_edges("forward")
+        b, tb = rg.count_all_edges("forward")
+    assert np.array_equal(a, b) and ta == tb
+
+
+@pytest.mark.parametrize("case", ["selfloop", "unsorted", "asym", "range"])
+def test_malformed_upload_rejected(ctx, case):
+    from acctricnt_amd import _ffi
+
+    off = np.array([0, 2, 4, 6], np.uint32)
+    adj = {"selfloop": [0, 1, 0, 2, 1, 2], "unsorted": [2, 1, 0, 2, 0, 1],
+           "asym": [1, 2, 0, 2, 1, 0], "range": [1, 5, 0, 2, 0, 1]}[case]
+    if case == "asym":
+        adj = [1, 2, 0, 2, 0, 0]
+    g = G.CSRGraph(off, np.array(adj, np.int32))
+    with pytest.raises(_ffi.AccError) as e:
+        ctx.upload(g)
+    assert e.value.code == -2
+
+
+def test_full_size_rmat22_properties(ctx):
+    """R-MAT scale 22 / ef 16: symmetry, sum = 6T, T equal to the oracle's, and
+    bit-exact counts on 200k uniformly sampled u<v slots and their reverse slots."""
+    g = G.gen_rmat(22, 16, seed=1)
+    print(f"[rmat22] n={g.n} m={g.m}", flush=True)
+    with ctx.upload(g) as rg:
+        cnt, t = rg.count_all_edges("forward")
+    rev = g.reverse_slots()
+    assert np.array_equal(cnt, cnt[rev])
+    assert int((cnt.astype(np.int64) - 2).sum()) == 6 * t
+    assert t == oracle.triangle_count(g)
+    rng = np.random.default_rng(0)
+    src = g.sources()
+    up = np.nonzero(src < g.edge_dst)[0]
+    sel = rng.choice(up, 200_000, replace=False)
+    ref = np.full(g.m, -1, np.int32)
+    oracle.cn_merge_slots(g, ref, sel, src[sel])
+    # slots written by the sample: the sampled u<v slots and their reverse slots
+    mask = ref >= 0
+    assert mask.sum() > 0
+    assert np.array_equal(cnt[mask], ref[mask])
