@@ -9,16 +9,10 @@ RapidsAtHKUST/AccTriCnt.  Native code:
 * ``libacc_graph.so`` host CSR utilities (include/acc_graph.h).
 
 Python modules: :mod:`.graph` (CSR, generators, on-disk format) and
-:mod:`.cn` (contexts, resident graphs, counting).
+:mod:`.cn` (contexts, resident graphs, counting).  Importing the package
+loads no native library; the first call that needs one loads it and fails
+loudly if it is not built.
 """
-from . import graph  # noqa: F401
+from . import cn, graph  # noqa: F401
 
 __all__ = ["graph", "cn"]
-
-
-def __getattr__(name):
-    if name == "cn":
-        from . import cn as _cn
-
-        return _cn
-    raise AttributeError(name)

@@ -12,6 +12,7 @@ instead of exiting the process.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -48,6 +49,7 @@ class Context:
         check_cn(self._lib.acc_ctx_create(device, C.byref(h)))
         self._h = h
         self.device = device
+        self._graphs = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -60,9 +62,13 @@ class Context:
         return self._lib.acc_ctx_stream(self._h) or 0
 
     def upload(self, graph, validate: bool = True) -> "ResidentGraph":
-        return ResidentGraph(self, graph, validate)
+        rg = ResidentGraph(self, graph, validate)
+        self._graphs.add(rg)
+        return rg
 
     def close(self) -> None:
+        for rg in list(self._graphs):
+            rg.release()
         if self._h:
             self._lib.acc_ctx_destroy(self._h)
             self._h = None

@@ -141,6 +141,7 @@ __global__ __launch_bounds__(kWG) void k_forward_count(const uint32_t *__restric
       const unsigned long long mlt = __ballot(lt);
       if (mlt != ~0ull) {
         p = base + static_cast<uint32_t>(__builtin_ctzll(~mlt));
+        if (p >= ce) p = ~0u;  // b absent from N(c): only possible for an unvalidated, asymmetric CSR
         break;
       }
     }
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(256) void k_combine(int32_t *__restrict__ cnt, cons
   unsigned long long s = 0;
   for (uint64_t i = tid; i < m; i += stride) {
     const uint32_t p = rev[i];
-    if (p != ~0u) {
+    if (p < m) {
       int c = cnt[i] + cnt[p];
       s += static_cast<unsigned long long>(c);
       c += plus;

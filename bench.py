@@ -72,6 +72,28 @@ def cpu_baseline(g, target_s: float):
                       f"merge-counted by the oracle in {t:.2f} s; full-graph estimate {t * up.size / k:.1f} s"}
 
 
+def calibration_copy(mib: int) -> None:
+    """One device-to-device copy of mib MiB through the HIP runtime: the plain
+    copy dispatch whose FETCH_SIZE / WRITE_SIZE counter readings scale the
+    counters of the counting kernels to bytes."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    size = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(size)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(size))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(b, a, C.c_size_t(size), 3) != 0:  # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] calibration copy of {mib} MiB done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +103,8 @@ def main():
     ap.add_argument("--path", default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the counting path runs on a single MI355X "
@@ -100,6 +124,8 @@ def main():
           f"B_alg={b_alg / 1e9:.2f} GB gen={t_gen:.1f}s", file=sys.stderr, flush=True)
 
     ctx = cn.Context(0)
+    if args.calibrate_copy:
+        calibration_copy(args.calibrate_copy)
     t0 = time.perf_counter()
     rg = ctx.upload(g)
     t_up = time.perf_counter() - t0
