@@ -53,9 +53,11 @@ struct acc_graph {
   uint32_t *d_rev = nullptr;            // scratch: reverse slot of every upper slot, ~0 elsewhere
   unsigned long long *d_sum = nullptr;  // sum over upper slots of the raw counts (= 3T)
   int32_t *d_flags = nullptr;           // validation flags
-  // forward plan
-  FwdUnit *d_fwd_small = nullptr;  // |N<(b)| fits the LDS table
-  uint32_t n_fwd_small = 0;
+  // forward plan, binned by degree skew
+  FwdUnit *d_fwd_wave = nullptr;  // few upper slots, small N<(b): one wave per source
+  uint32_t n_fwd_wave = 0;
+  FwdUnit *d_fwd_block = nullptr;  // one workgroup per chunk of upper slots, N<(b) in LDS
+  uint32_t n_fwd_block = 0;
   FwdUnit *d_fwd_big = nullptr;  // |N<(b)| beyond the LDS table: global-memory lookups
   uint32_t n_fwd_big = 0;
   uint64_t device_bytes = 0;
@@ -68,10 +70,15 @@ struct acc_graph {
 
 // Kernel launchers (k_forward.hip).
 namespace accfwd {
-constexpr int kWG = 256;
-constexpr int kTableBits = 12;                 // LDS hash table of 4096 keys
+constexpr int kWG = 512;                              // threads per workgroup (8 waves)
+constexpr int kWaves = kWG / 64;
+constexpr int kTableBits = 12;                        // workgroup LDS hash table of 4096 keys
 constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
-constexpr uint32_t kChunk = 512;               // upper slots per work unit
+constexpr uint32_t kChunk = 512;                      // upper slots per workgroup unit
+constexpr int kWaveTableBits = 9;                     // wave-private table of 512 keys
+constexpr uint32_t kWaveCap = 1u << (kWaveTableBits - 1);  // |N<(b)| up to 256 in a wave unit
+constexpr uint32_t kWaveSlots = 64;                   // upper slots of a wave unit
+constexpr int kBigWG = 256;
 int launch_init(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);

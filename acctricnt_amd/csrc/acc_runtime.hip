@@ -32,17 +32,21 @@ void free_graph(acc_graph *g) {
   DeviceGuard guard(g->ctx->device);
   for (void *p : {static_cast<void *>(g->d_off), static_cast<void *>(g->d_adj), static_cast<void *>(g->d_cnt),
                   static_cast<void *>(g->d_rev), static_cast<void *>(g->d_sum), static_cast<void *>(g->d_flags),
-                  static_cast<void *>(g->d_fwd_small), static_cast<void *>(g->d_fwd_big)})
+                  static_cast<void *>(g->d_fwd_wave), static_cast<void *>(g->d_fwd_block),
+                  static_cast<void *>(g->d_fwd_big)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
   delete g;
 }
 
-// Forward plan: for every source b with upper slots (neighbors > b), units of
-// at most kChunk upper slots; sources whose |N<(b)| exceeds the LDS table go
-// to the global-memory variant.
-void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &small,
-                        std::vector<FwdUnit> &big, uint32_t &max_low, uint32_t &max_deg) {
+// Forward plan, binned by degree skew: a source b with upper slots (neighbors
+// > b) becomes one wave unit when it has at most kWaveSlots upper slots and
+// |N<(b)| <= kWaveCap, otherwise workgroup units of at most kChunk upper
+// slots; sources whose |N<(b)| exceeds the workgroup table go to the
+// global-memory variant.
+void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &wave,
+                        std::vector<FwdUnit> &block, std::vector<FwdUnit> &big, uint32_t &max_low,
+                        uint32_t &max_deg) {
   max_low = 0;
   max_deg = 0;
   for (uint32_t b = 0; b < n; b++) {
@@ -51,9 +55,15 @@ void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std
     const uint32_t nl = hs - off[b];
     max_low = std::max(max_low, nl);
     max_deg = std::max(max_deg, off[b + 1] - off[b]);
+    const uint32_t nup = off[b + 1] - hs;
+    if (nup == 0) continue;
+    if (nup <= accfwd::kWaveSlots && nl <= accfwd::kWaveCap) {
+      wave.push_back(FwdUnit{static_cast<int32_t>(b), hs, off[b + 1], nl});
+      continue;
+    }
     for (uint32_t j0 = hs; j0 < off[b + 1]; j0 += accfwd::kChunk) {
       FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + accfwd::kChunk, off[b + 1]), nl};
-      (nl <= accfwd::kLdsCap ? small : big).push_back(u);
+      (nl <= accfwd::kLdsCap ? block : big).push_back(u);
     }
   }
 }
@@ -179,19 +189,19 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
     if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
-  std::vector<FwdUnit> small, big;
-  build_forward_plan(n, node_off, edge_dst, small, big, g->max_low, g->max_deg);
-  g->n_fwd_small = static_cast<uint32_t>(small.size());
-  g->n_fwd_big = static_cast<uint32_t>(big.size());
-  if (!small.empty()) {
-    if ((rc = dev_alloc(g, &g->d_fwd_small, small.size()))) return bail(rc);
-    e = hipMemcpyAsync(g->d_fwd_small, small.data(), sizeof(FwdUnit) * small.size(), hipMemcpyHostToDevice,
-                       ctx->stream);
-    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
-  }
-  if (!big.empty()) {
-    if ((rc = dev_alloc(g, &g->d_fwd_big, big.size()))) return bail(rc);
-    e = hipMemcpyAsync(g->d_fwd_big, big.data(), sizeof(FwdUnit) * big.size(), hipMemcpyHostToDevice, ctx->stream);
+  std::vector<FwdUnit> wave, block, big;
+  build_forward_plan(n, node_off, edge_dst, wave, block, big, g->max_low, g->max_deg);
+  struct {
+    std::vector<FwdUnit> *v;
+    FwdUnit **d;
+    uint32_t *count;
+  } lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave}, {&block, &g->d_fwd_block, &g->n_fwd_block},
+               {&big, &g->d_fwd_big, &g->n_fwd_big}};
+  for (auto &l : lists) {
+    *l.count = static_cast<uint32_t>(l.v->size());
+    if (l.v->empty()) continue;
+    if ((rc = dev_alloc(g, l.d, l.v->size()))) return bail(rc);
+    e = hipMemcpyAsync(*l.d, l.v->data(), sizeof(FwdUnit) * l.v->size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
   }
   e = hipStreamSynchronize(ctx->stream);

@@ -22,7 +22,9 @@ namespace {
 using accfwd::kChunk;
 using accfwd::kLdsCap;
 using accfwd::kTableBits;
+using accfwd::kBigWG;
 using accfwd::kWG;
+using accfwd::kWaves;
 
 __device__ __forceinline__ uint32_t hash_slot(int32_t a, int bits) {
   return (static_cast<uint32_t>(a) * 0x9E3779B1u) >> (32 - bits);
@@ -58,44 +60,206 @@ __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint32_
   if (tid == 0) *sum = 0;
 }
 
-// One workgroup per unit (source b, upper slots [j0, j1)).  kLds: N<(b) in an
-// LDS hash table (|N<(b)| <= kLdsCap); otherwise binary search of N<(b) in
-// global memory and direct atomics for the (b,a) role.
-template <bool kLds>
-__global__ __launch_bounds__(kWG) void k_forward_count(const uint32_t *__restrict__ off,
-                                                       const int32_t *__restrict__ adj,
-                                                       const FwdUnit *__restrict__ units,
-                                                       int32_t *cnt, uint32_t *__restrict__ rev) {
-  constexpr int kT = kLds ? (1 << kTableBits) : 1;
+// Makes the wave's earlier LDS writes visible to all of its lanes before
+// later LDS reads (wave-private tables need no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int table_bits_for(uint32_t nl) {
+  int bits = 6;
+  while ((1u << bits) < 2 * nl) bits++;
+  return bits;
+}
+
+// Open-addressing set of N<(b) in LDS: key = vertex id, value = its position in N<(b).
+__device__ __forceinline__ int lds_find(const int32_t *keys, const uint16_t *kpos, int bits, uint32_t mask,
+                                        int32_t a) {
+  uint32_t h = hash_slot(a, bits);
+  for (;;) {
+    const int32_t k = keys[h];
+    if (k == a) return kpos[h];
+    if (k == -1) return -1;
+    h = (h + 1) & mask;
+  }
+}
+
+// Inserts N<(b) = adj[sb, sb+nl) with a stride of `step` threads starting at `t`.
+__device__ __forceinline__ void lds_insert(int32_t *keys, uint16_t *kpos, int bits, uint32_t mask,
+                                           const int32_t *__restrict__ adj, uint32_t sb, uint32_t nl, int t,
+                                           int step) {
+  for (uint32_t i = t; i < nl; i += step) {
+    const int32_t a = adj[sb + i];
+    uint32_t h = hash_slot(a, bits);
+    while (atomicCAS(&keys[h], -1, a) != -1) h = (h + 1) & mask;
+    kpos[h] = static_cast<uint16_t>(i);
+  }
+}
+
+// Scans N(c) from its start up to b, two 64-element chunks per round.  Every
+// element a < b found in N<(b) is a triangle {a, b, c}: count it for (b,c) in
+// the returned total, for (b,a) through ctr[pos] (LDS), and for (c,a) through
+// an atomic on the element's own slot.  *p receives the position of b in N(c).
+__device__ __forceinline__ int scan_probe_lds(const int32_t *__restrict__ adj, int32_t *cnt, const int32_t *keys,
+                                              const uint16_t *kpos, int32_t *ctr, int bits, uint32_t mask,
+                                              int32_t b, uint32_t cb, uint32_t ce, int lane, uint32_t *p) {
+  int reg = 0;
+  for (uint32_t base = cb;; base += 128) {
+    const uint32_t i0 = base + lane, i1 = i0 + 64;
+    const int32_t a0 = i0 < ce ? adj[i0] : INT32_MAX;
+    const int32_t a1 = i1 < ce ? adj[i1] : INT32_MAX;
+    const bool lt0 = a0 < b, lt1 = a1 < b;
+    if (lt0) {
+      const int pos = lds_find(keys, kpos, bits, mask, a0);
+      if (pos >= 0) {
+        reg++;
+        atomicAdd(&cnt[i0], 1);
+        atomicAdd(&ctr[pos], 1);
+      }
+    }
+    if (lt1) {
+      const int pos = lds_find(keys, kpos, bits, mask, a1);
+      if (pos >= 0) {
+        reg++;
+        atomicAdd(&cnt[i1], 1);
+        atomicAdd(&ctr[pos], 1);
+      }
+    }
+    const unsigned long long m0 = __ballot(lt0);
+    if (m0 != ~0ull) {
+      *p = base + static_cast<uint32_t>(__builtin_ctzll(~m0));
+      break;
+    }
+    const unsigned long long m1 = __ballot(lt1);
+    if (m1 != ~0ull) {
+      *p = base + 64 + static_cast<uint32_t>(__builtin_ctzll(~m1));
+      break;
+    }
+  }
+  if (*p >= ce) *p = ~0u;  // b absent from N(c): only possible for an unvalidated, asymmetric CSR
+  return reg;
+}
+
+// Wave units: sources with at most kWaveSlots upper slots and |N<(b)| <= kWaveCap.
+// Each wave of the workgroup takes its own source with a wave-private table.
+__global__ __launch_bounds__(kWG) void k_fwd_wave(const uint32_t *__restrict__ off,
+                                                  const int32_t *__restrict__ adj,
+                                                  const FwdUnit *__restrict__ units, uint32_t n_units,
+                                                  int32_t *cnt, uint32_t *__restrict__ rev) {
+  constexpr int kT = 1 << accfwd::kWaveTableBits;
+  __shared__ int32_t keys_all[accfwd::kWaves][kT];
+  __shared__ uint16_t kpos_all[accfwd::kWaves][kT];
+  __shared__ int32_t ctr_all[accfwd::kWaves][accfwd::kWaveCap];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const uint32_t ui = blockIdx.x * accfwd::kWaves + wave;
+  if (ui >= n_units) return;  // this kernel has no workgroup barrier
+  int32_t *keys = keys_all[wave];
+  uint16_t *kpos = kpos_all[wave];
+  int32_t *ctr = ctr_all[wave];
+  const FwdUnit u = units[ui];
+  const int32_t b = u.b;
+  const uint32_t sb = off[b];
+  const uint32_t nl = u.nlow;
+  const uint32_t nup = u.j1 - u.j0;
+  const int bits = table_bits_for(nl);
+  const uint32_t mask = (1u << bits) - 1;
+  // neighbor metadata, one upper slot per lane
+  const int32_t my_c = lane < static_cast<int>(nup) ? adj[u.j0 + lane] : 0;
+  const uint32_t my_cb = lane < static_cast<int>(nup) ? off[my_c] : 0;
+  const uint32_t my_ce = lane < static_cast<int>(nup) ? off[my_c + 1] : 0;
+  for (uint32_t i = lane; i <= mask; i += 64) keys[i] = -1;
+  for (uint32_t i = lane; i < nl; i += 64) ctr[i] = 0;
+  wave_lds_sync();
+  lds_insert(keys, kpos, bits, mask, adj, sb, nl, lane, 64);
+  wave_lds_sync();
+  for (uint32_t k = 0; k < nup; k++) {
+    const uint32_t cb = __builtin_amdgcn_readfirstlane(__shfl(my_cb, k));
+    const uint32_t ce = __builtin_amdgcn_readfirstlane(__shfl(my_ce, k));
+    uint32_t p = ~0u;
+    int reg = scan_probe_lds(adj, cnt, keys, kpos, ctr, bits, mask, b, cb, ce, lane, &p);
+    reg = wave_sum_i32(reg);
+    if (lane == 0) {
+      cnt[u.j0 + k] = reg;
+      rev[u.j0 + k] = p;
+    }
+  }
+  wave_lds_sync();
+  for (uint32_t i = lane; i < nl; i += 64) {
+    const int v = ctr[i];
+    if (v) atomicAdd(&cnt[sb + i], v);
+  }
+}
+
+// Workgroup units: one chunk of at most kChunk upper slots of a source whose
+// N<(b) fits the workgroup table.
+__global__ __launch_bounds__(kWG) void k_fwd_block(const uint32_t *__restrict__ off,
+                                                   const int32_t *__restrict__ adj,
+                                                   const FwdUnit *__restrict__ units, int32_t *cnt,
+                                                   uint32_t *__restrict__ rev) {
+  constexpr int kT = 1 << kTableBits;
   __shared__ int32_t keys[kT];
   __shared__ uint16_t kpos[kT];
-  __shared__ int32_t ctr[kLds ? kLdsCap : 1];
-
+  __shared__ int32_t ctr[kLdsCap];
+  __shared__ uint32_t s_cb[kChunk];
+  __shared__ uint32_t s_ce[kChunk];
   const FwdUnit u = units[blockIdx.x];
   const int32_t b = u.b;
   const uint32_t sb = off[b];
   const uint32_t nl = u.nlow;
+  const uint32_t nup = u.j1 - u.j0;
   const int tid = threadIdx.x;
-  int bits = 6;
-  while ((1u << bits) < 2 * nl) bits++;
+  const int bits = table_bits_for(nl);
   const uint32_t mask = (1u << bits) - 1;
-
-  if (kLds) {
-    for (uint32_t i = tid; i <= mask; i += kWG) keys[i] = -1;
-    for (uint32_t i = tid; i < nl; i += kWG) ctr[i] = 0;
-    __syncthreads();
-    for (uint32_t i = tid; i < nl; i += kWG) {
-      const int32_t a = adj[sb + i];
-      uint32_t h = hash_slot(a, bits);
-      while (atomicCAS(&keys[h], -1, a) != -1) h = (h + 1) & mask;
-      kpos[h] = static_cast<uint16_t>(i);
-    }
-    __syncthreads();
+  for (uint32_t i = tid; i <= mask; i += kWG) keys[i] = -1;
+  for (uint32_t i = tid; i < nl; i += kWG) ctr[i] = 0;
+  for (uint32_t i = tid; i < nup; i += kWG) {
+    const int32_t c = adj[u.j0 + i];
+    s_cb[i] = off[c];
+    s_ce[i] = off[c + 1];
   }
-
-  const int wave = tid >> 6;
+  __syncthreads();
+  lds_insert(keys, kpos, bits, mask, adj, sb, nl, tid, kWG);
+  __syncthreads();
+  // Static round-robin of neighbors over waves.  The loop index must stay
+  // wave-uniform and must not be produced by a lane-0-only branch: a work
+  // counter taken by lane 0 and broadcast lets the compiler rotate the loop so
+  // that lanes 1..63 re-enter it without lane 0 and never exit.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
-  for (uint32_t j = u.j0 + wave; j < u.j1; j += kWG / 64) {
+  for (uint32_t k = wave; k < nup; k += kWaves) {
+    uint32_t p = ~0u;
+    const uint32_t cb = __builtin_amdgcn_readfirstlane(s_cb[k]);
+    const uint32_t ce = __builtin_amdgcn_readfirstlane(s_ce[k]);
+    int reg = scan_probe_lds(adj, cnt, keys, kpos, ctr, bits, mask, b, cb, ce, lane, &p);
+    reg = wave_sum_i32(reg);
+    if (lane == 0) {
+      cnt[u.j0 + k] = reg;
+      rev[u.j0 + k] = p;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nl; i += kWG) {
+    const int v = ctr[i];
+    if (v) atomicAdd(&cnt[sb + i], v);
+  }
+}
+
+// Sources whose N<(b) exceeds the workgroup table: membership by binary
+// search of N<(b) in global memory, (b,a) counted by direct atomics.
+__global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__restrict__ off,
+                                                            const int32_t *__restrict__ adj,
+                                                            const FwdUnit *__restrict__ units, int32_t *cnt,
+                                                            uint32_t *__restrict__ rev) {
+  const FwdUnit u = units[blockIdx.x];
+  const int32_t b = u.b;
+  const uint32_t sb = off[b];
+  const uint32_t nl = u.nlow;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t j = u.j0 + wave; j < u.j1; j += accfwd::kBigWG / 64) {
     const int32_t c = adj[j];
     const uint32_t cb = off[c];
     const uint32_t ce = off[c + 1];
@@ -106,42 +270,24 @@ __global__ __launch_bounds__(kWG) void k_forward_count(const uint32_t *__restric
       const int32_t a = idx < ce ? adj[idx] : INT32_MAX;
       const bool lt = a < b;
       if (lt) {
-        int pos = -1;
-        if (kLds) {
-          uint32_t h = hash_slot(a, bits);
-          for (;;) {
-            const int32_t k = keys[h];
-            if (k == a) {
-              pos = kpos[h];
-              break;
-            }
-            if (k == -1) break;
-            h = (h + 1) & mask;
-          }
-        } else {
-          uint32_t lo = 0, hi = nl;
-          while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (adj[sb + mid] < a)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          if (lo < nl && adj[sb + lo] == a) pos = static_cast<int>(lo);
+        uint32_t lo = 0, hi = nl;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (adj[sb + mid] < a)
+            lo = mid + 1;
+          else
+            hi = mid;
         }
-        if (pos >= 0) {
+        if (lo < nl && adj[sb + lo] == a) {
           reg++;
           atomicAdd(&cnt[idx], 1);
-          if (kLds)
-            atomicAdd(&ctr[pos], 1);
-          else
-            atomicAdd(&cnt[sb + pos], 1);
+          atomicAdd(&cnt[sb + lo], 1);
         }
       }
       const unsigned long long mlt = __ballot(lt);
       if (mlt != ~0ull) {
         p = base + static_cast<uint32_t>(__builtin_ctzll(~mlt));
-        if (p >= ce) p = ~0u;  // b absent from N(c): only possible for an unvalidated, asymmetric CSR
+        if (p >= ce) p = ~0u;
         break;
       }
     }
@@ -149,13 +295,6 @@ __global__ __launch_bounds__(kWG) void k_forward_count(const uint32_t *__restric
     if (lane == 0) {
       cnt[j] = reg;
       rev[j] = p;
-    }
-  }
-  if (kLds) {
-    __syncthreads();
-    for (uint32_t i = tid; i < nl; i += kWG) {
-      const int v = ctr[i];
-      if (v) atomicAdd(&cnt[sb + i], v);
     }
   }
 }
@@ -239,14 +378,21 @@ int launch_init(acc_graph *g) {
 }
 
 int launch_count(acc_graph *g) {
-  if (g->n_fwd_small) {
-    hipLaunchKernelGGL(k_forward_count<true>, dim3(g->n_fwd_small), dim3(kWG), 0, g->ctx->stream, g->d_off,
-                       g->d_adj, g->d_fwd_small, g->d_cnt, g->d_rev);
+  hipStream_t st = g->ctx->stream;
+  if (g->n_fwd_block) {
+    hipLaunchKernelGGL(k_fwd_block, dim3(g->n_fwd_block), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
+                       g->d_cnt, g->d_rev);
+    ACC_HIP_TRY(hipGetLastError());
+  }
+  if (g->n_fwd_wave) {
+    const unsigned grid = (g->n_fwd_wave + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL(k_fwd_wave, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_wave,
+                       g->n_fwd_wave, g->d_cnt, g->d_rev);
     ACC_HIP_TRY(hipGetLastError());
   }
   if (g->n_fwd_big) {
-    hipLaunchKernelGGL(k_forward_count<false>, dim3(g->n_fwd_big), dim3(kWG), 0, g->ctx->stream, g->d_off,
-                       g->d_adj, g->d_fwd_big, g->d_cnt, g->d_rev);
+    hipLaunchKernelGGL(k_fwd_big, dim3(g->n_fwd_big), dim3(kBigWG), 0, st, g->d_off, g->d_adj, g->d_fwd_big,
+                       g->d_cnt, g->d_rev);
     ACC_HIP_TRY(hipGetLastError());
   }
   return ACC_OK;
