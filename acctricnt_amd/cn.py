@@ -25,6 +25,9 @@ PATH_BITMAP = 2
 PATH_MERGE = 4
 PATHS = {"auto": PATH_AUTO, "forward": PATH_FORWARD, "bitmap": PATH_BITMAP, "merge": PATH_MERGE}
 
+PARAM_FWD_UNROLL = 1
+PARAMS = {"fwd_unroll": PARAM_FWD_UNROLL}
+
 CN_RAW = 0x1
 CN_TIMING = 0x2
 UPLOAD_NO_VALIDATE = 0x1
@@ -145,6 +148,16 @@ class ResidentGraph:
             check_cn(k)
         keys = names.value.decode().split(";") if k else []
         return {key: float(ms[i]) for i, key in enumerate(keys)}
+
+    def set_param(self, name, value: int) -> None:
+        key = PARAMS[name] if isinstance(name, str) else int(name)
+        check_cn(self._lib.acc_graph_set_param(self._h, key, int(value)))
+
+    def get_param(self, name) -> int:
+        key = PARAMS[name] if isinstance(name, str) else int(name)
+        v = C.c_int64()
+        check_cn(self._lib.acc_graph_get_param(self._h, key, C.byref(v)))
+        return int(v.value)
 
     def device_counts_ptr(self) -> int:
         return self._lib.acc_cn_device_counts(self._h) or 0

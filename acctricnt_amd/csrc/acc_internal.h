@@ -61,6 +61,8 @@ struct acc_graph {
   FwdUnit *d_fwd_big = nullptr;  // |N<(b)| beyond the LDS table: global-memory lookups
   uint32_t n_fwd_big = 0;
   uint64_t device_bytes = 0;
+  // tuning parameters (acc_graph_set_param)
+  int fwd_unroll = 2;
   // last results / timing
   bool have_result = false;
   std::vector<hipEvent_t> events;
@@ -75,12 +77,15 @@ constexpr int kWaves = kWG / 64;
 constexpr int kTableBits = 12;                        // workgroup LDS hash table of 4096 keys
 constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
 constexpr uint32_t kChunk = 512;                      // upper slots per workgroup unit
+constexpr uint32_t kBitmapIds = 32u << kTableBits;    // sources b <= 2^17: N<(b) as an LDS bitmap of [0, b)
+constexpr uint32_t kBitmapMaxLow = 65535;             // word prefixes are 16-bit
 constexpr int kWaveTableBits = 9;                     // wave-private table of 512 keys
 constexpr uint32_t kWaveCap = 1u << (kWaveTableBits - 1);  // |N<(b)| up to 256 in a wave unit
 constexpr uint32_t kWaveSlots = 64;                   // upper slots of a wave unit
 constexpr int kBigWG = 256;
 int launch_init(acc_graph *g);
 int launch_count(acc_graph *g);
+int launch_count_big(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);
 int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
                     hipStream_t s);

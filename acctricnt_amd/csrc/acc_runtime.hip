@@ -42,8 +42,8 @@ void free_graph(acc_graph *g) {
 // Forward plan, binned by degree skew: a source b with upper slots (neighbors
 // > b) becomes one wave unit when it has at most kWaveSlots upper slots and
 // |N<(b)| <= kWaveCap, otherwise workgroup units of at most kChunk upper
-// slots; sources whose |N<(b)| exceeds the workgroup table go to the
-// global-memory variant.
+// slots (N<(b) as an LDS bitmap for b <= kBitmapIds, else an LDS hash
+// table); sources whose N<(b) fits neither go to the global-memory variant.
 void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &wave,
                         std::vector<FwdUnit> &block, std::vector<FwdUnit> &big, uint32_t &max_low,
                         uint32_t &max_deg) {
@@ -63,7 +63,8 @@ void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std
     }
     for (uint32_t j0 = hs; j0 < off[b + 1]; j0 += accfwd::kChunk) {
       FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + accfwd::kChunk, off[b + 1]), nl};
-      (nl <= accfwd::kLdsCap ? block : big).push_back(u);
+      const bool lds = nl <= accfwd::kLdsCap || (b <= accfwd::kBitmapIds && nl <= accfwd::kBitmapMaxLow);
+      (lds ? block : big).push_back(u);
     }
   }
 }
@@ -234,7 +235,11 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   ACC_TRY(accfwd::launch_init(g));
   ACC_TRY(record(g, timing, "k_init"));
   ACC_TRY(accfwd::launch_count(g));
-  ACC_TRY(record(g, timing, "k_forward_count"));
+  ACC_TRY(record(g, timing, "k_fwd_count"));
+  if (g->n_fwd_big) {
+    ACC_TRY(accfwd::launch_count_big(g));
+    ACC_TRY(record(g, timing, "k_fwd_big"));
+  }
   ACC_TRY(accfwd::launch_combine(g, (flags & ACC_CN_RAW) ? 0 : 2));
   ACC_TRY(record(g, timing, "k_combine"));
   g->have_result = true;
@@ -281,6 +286,29 @@ int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap)
     names[names_cap - 1] = '\0';
   }
   return k;
+}
+
+int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  switch (param) {
+    case ACC_PARAM_FWD_UNROLL:
+      if (value != 1 && value != 2 && value != 4) return acc_set_error(ACC_EINVAL, "fwd_unroll must be 1, 2 or 4");
+      g->fwd_unroll = static_cast<int>(value);
+      return ACC_OK;
+    default:
+      return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
+  }
+}
+
+int acc_graph_get_param(const acc_graph *g, int param, int64_t *value) {
+  if (!g || !value) return acc_set_error(ACC_EINVAL, "null argument");
+  switch (param) {
+    case ACC_PARAM_FWD_UNROLL:
+      *value = g->fwd_unroll;
+      return ACC_OK;
+    default:
+      return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
+  }
 }
 
 int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles) {

@@ -98,68 +98,177 @@ __device__ __forceinline__ void lds_insert(int32_t *keys, uint16_t *kpos, int bi
   }
 }
 
-// Scans N(c) from its start up to b, two 64-element chunks per round.  Every
+// Membership of N<(b) as an open-addressing hash table in LDS.
+struct HashSet {
+  const int32_t *keys;
+  const uint16_t *kpos;
+  int bits;
+  uint32_t mask;
+  __device__ __forceinline__ int find(int32_t a) const { return lds_find(keys, kpos, bits, mask, a); }
+};
+
+// Membership of N<(b) ⊂ [0, b) as an exact LDS bitmap over [0, b) with the
+// position of a in N<(b) recovered from the index of the first element of
+// a's 32-bit word (pre) plus a popcount: one LDS read decides a probe.
+struct BitSet {
+  const uint32_t *bm;
+  const uint16_t *pre;
+  __device__ __forceinline__ int find(int32_t a) const {
+    const uint32_t w = bm[a >> 5];
+    const uint32_t bit = 1u << (a & 31);
+    if (!(w & bit)) return -1;
+    return pre[a >> 5] + __popc(w & (bit - 1));
+  }
+};
+
+// Scans N(c) from its start up to b, U 64-element chunks per round (the loads
+// of a round are independent, so U sets the memory-level parallelism).  Every
 // element a < b found in N<(b) is a triangle {a, b, c}: count it for (b,c) in
-// the returned total, for (b,a) through ctr[pos] (LDS), and for (c,a) through
-// an atomic on the element's own slot.  *p receives the position of b in N(c).
-__device__ __forceinline__ int scan_probe_lds(const int32_t *__restrict__ adj, int32_t *cnt, const int32_t *keys,
-                                              const uint16_t *kpos, int32_t *ctr, int bits, uint32_t mask,
-                                              int32_t b, uint32_t cb, uint32_t ce, int lane, uint32_t *p) {
+// the returned total, for (b,a) through ctr[pos] (LDS, positions below
+// ctr_cap) or an atomic on the lower slot sb+pos, and for (c,a) through an
+// atomic on the element's own slot.  *p receives the position of b in N(c).
+template <int U, class Set>
+__device__ __forceinline__ int scan_probe(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
+                                          int32_t *ctr, uint32_t ctr_cap, uint32_t sb, int32_t b, uint32_t cb,
+                                          uint32_t ce, int lane, uint32_t *p) {
   int reg = 0;
-  for (uint32_t base = cb;; base += 128) {
-    const uint32_t i0 = base + lane, i1 = i0 + 64;
-    const int32_t a0 = i0 < ce ? adj[i0] : INT32_MAX;
-    const int32_t a1 = i1 < ce ? adj[i1] : INT32_MAX;
-    const bool lt0 = a0 < b, lt1 = a1 < b;
-    if (lt0) {
-      const int pos = lds_find(keys, kpos, bits, mask, a0);
-      if (pos >= 0) {
-        reg++;
-        atomicAdd(&cnt[i0], 1);
-        atomicAdd(&ctr[pos], 1);
+  for (uint32_t base = cb;; base += 64 * U) {
+    int32_t a[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t i = base + 64 * q + lane;
+      a[q] = i < ce ? adj[i] : INT32_MAX;
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      if (a[q] < b) {
+        const int pos = set.find(a[q]);
+        if (pos >= 0) {
+          reg++;
+          atomicAdd(&cnt[base + 64 * q + lane], 1);
+          if (static_cast<uint32_t>(pos) < ctr_cap)
+            atomicAdd(&ctr[pos], 1);
+          else
+            atomicAdd(&cnt[sb + pos], 1);
+        }
       }
     }
-    if (lt1) {
-      const int pos = lds_find(keys, kpos, bits, mask, a1);
-      if (pos >= 0) {
-        reg++;
-        atomicAdd(&cnt[i1], 1);
-        atomicAdd(&ctr[pos], 1);
+    bool done = false;
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const unsigned long long m = __ballot(a[q] < b);
+      if (m != ~0ull) {
+        *p = base + 64 * q + static_cast<uint32_t>(__builtin_ctzll(~m));
+        done = true;
+        break;
       }
     }
-    const unsigned long long m0 = __ballot(lt0);
-    if (m0 != ~0ull) {
-      *p = base + static_cast<uint32_t>(__builtin_ctzll(~m0));
-      break;
-    }
-    const unsigned long long m1 = __ballot(lt1);
-    if (m1 != ~0ull) {
-      *p = base + 64 + static_cast<uint32_t>(__builtin_ctzll(~m1));
-      break;
-    }
+    if (done) break;
   }
   if (*p >= ce) *p = ~0u;  // b absent from N(c): only possible for an unvalidated, asymmetric CSR
   return reg;
 }
 
-// Wave units: sources with at most kWaveSlots upper slots and |N<(b)| <= kWaveCap.
-// Each wave of the workgroup takes its own source with a wave-private table.
-__global__ __launch_bounds__(kWG) void k_fwd_wave(const uint32_t *__restrict__ off,
-                                                  const int32_t *__restrict__ adj,
-                                                  const FwdUnit *__restrict__ units, uint32_t n_units,
-                                                  int32_t *cnt, uint32_t *__restrict__ rev) {
-  constexpr int kT = 1 << accfwd::kWaveTableBits;
-  __shared__ int32_t keys_all[accfwd::kWaves][kT];
-  __shared__ uint16_t kpos_all[accfwd::kWaves][kT];
-  __shared__ int32_t ctr_all[accfwd::kWaves][accfwd::kWaveCap];
+constexpr int kWT = 1 << accfwd::kWaveTableBits;
+constexpr int kBT = 1 << kTableBits;
+constexpr size_t kBlockLdsBytes = 4 * kBT + 2 * kBT + 4 * kLdsCap + 8 * kChunk;
+constexpr size_t kWaveLdsBytes = kWaves * (4 * kWT + 2 * kWT + 4 * accfwd::kWaveCap);
+constexpr size_t kCountLdsBytes = kBlockLdsBytes > kWaveLdsBytes ? kBlockLdsBytes : kWaveLdsBytes;
+
+// Runs the neighbor loop of a workgroup unit once the set of N<(b) is built.
+template <int U, class Set>
+__device__ __forceinline__ void fwd_block_loop(const int32_t *__restrict__ adj, const FwdUnit u, int32_t *cnt,
+                                               uint32_t *__restrict__ rev, const Set &set, int32_t *ctr,
+                                               const uint32_t *s_cb, const uint32_t *s_ce, uint32_t sb,
+                                               uint32_t nup) {
+  // Static round-robin of neighbors over waves.  The loop index must stay
+  // wave-uniform and must not be produced by a lane-0-only branch: a work
+  // counter taken by lane 0 and broadcast lets the compiler rotate the loop so
+  // that lanes 1..63 re-enter it without lane 0 and never exit.
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const uint32_t ui = blockIdx.x * accfwd::kWaves + wave;
-  if (ui >= n_units) return;  // this kernel has no workgroup barrier
-  int32_t *keys = keys_all[wave];
-  uint16_t *kpos = kpos_all[wave];
-  int32_t *ctr = ctr_all[wave];
-  const FwdUnit u = units[ui];
+  for (uint32_t k = wave; k < nup; k += kWaves) {
+    uint32_t p = ~0u;
+    const uint32_t cb = __builtin_amdgcn_readfirstlane(s_cb[k]);
+    const uint32_t ce = __builtin_amdgcn_readfirstlane(s_ce[k]);
+    int reg = scan_probe<U>(adj, cnt, set, ctr, kLdsCap, sb, u.b, cb, ce, lane, &p);
+    reg = wave_sum_i32(reg);
+    if (lane == 0) {
+      cnt[u.j0 + k] = reg;
+      rev[u.j0 + k] = p;
+    }
+  }
+}
+
+// Workgroup unit: one chunk of at most kChunk upper slots of a source b.
+// N<(b) is an LDS bitmap when b <= kBitmapIds (|N<(b)| < 65536), otherwise an
+// LDS hash table (|N<(b)| <= kLdsCap).  Both layouts share the ctr and
+// neighbor-metadata regions of smem.
+template <int U>
+__device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                               const FwdUnit u, int32_t *cnt, uint32_t *__restrict__ rev,
+                                               unsigned char *smem) {
+  int32_t *keys = reinterpret_cast<int32_t *>(smem);  // hash keys, or bitmap words
+  int32_t *ctr = keys + kBT;
+  uint32_t *s_cb = reinterpret_cast<uint32_t *>(ctr + kLdsCap);
+  uint32_t *s_ce = s_cb + kChunk;
+  uint16_t *kpos = reinterpret_cast<uint16_t *>(s_ce + kChunk);  // hash positions, or word prefixes
+  const int32_t b = u.b;
+  const uint32_t sb = off[b];
+  const uint32_t nl = u.nlow;
+  const uint32_t nup = u.j1 - u.j0;
+  const int tid = threadIdx.x;
+  const bool bitmap = static_cast<uint32_t>(b) <= accfwd::kBitmapIds;
+  const uint32_t nctr = nl < kLdsCap ? nl : kLdsCap;
+  const int bits = bitmap ? 0 : table_bits_for(nl);
+  const uint32_t mask = bitmap ? 0 : (1u << bits) - 1;
+  const uint32_t nclear = bitmap ? (static_cast<uint32_t>(b) + 31) / 32 : mask + 1;
+  const int32_t fill = bitmap ? 0 : -1;
+  for (uint32_t i = tid; i < nclear; i += kWG) keys[i] = fill;
+  for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
+  for (uint32_t i = tid; i < nup; i += kWG) {
+    const int32_t c = adj[u.j0 + i];
+    s_cb[i] = off[c];
+    s_ce[i] = off[c + 1];
+  }
+  __syncthreads();
+  if (bitmap) {
+    uint32_t *bm = reinterpret_cast<uint32_t *>(keys);
+    for (uint32_t i = tid; i < nl; i += kWG) {
+      const int32_t a = adj[sb + i];
+      atomicOr(&bm[a >> 5], 1u << (a & 31));
+      if (i == 0 || (adj[sb + i - 1] >> 5) != (a >> 5)) kpos[a >> 5] = static_cast<uint16_t>(i);
+    }
+  } else {
+    lds_insert(keys, kpos, bits, mask, adj, sb, nl, tid, kWG);
+  }
+  __syncthreads();
+  if (bitmap) {
+    const BitSet set{reinterpret_cast<const uint32_t *>(keys), kpos};
+    fwd_block_loop<U>(adj, u, cnt, rev, set, ctr, s_cb, s_ce, sb, nup);
+  } else {
+    const HashSet set{keys, kpos, bits, mask};
+    fwd_block_loop<U>(adj, u, cnt, rev, set, ctr, s_cb, s_ce, sb, nup);
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < nctr; i += kWG) {
+    const int v = ctr[i];
+    if (v) atomicAdd(&cnt[sb + i], v);
+  }
+}
+
+// Wave unit: a source with at most kWaveSlots upper slots and
+// |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.  No
+// workgroup barrier may follow a call: waves without a unit return early.
+template <int U>
+__device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                              const FwdUnit u, int32_t *cnt, uint32_t *__restrict__ rev,
+                                              unsigned char *smem, int wave) {
+  int32_t *keys = reinterpret_cast<int32_t *>(smem) + wave * kWT;
+  int32_t *ctr = reinterpret_cast<int32_t *>(smem) + kWaves * kWT + wave * accfwd::kWaveCap;
+  uint16_t *kpos = reinterpret_cast<uint16_t *>(reinterpret_cast<int32_t *>(smem) + kWaves * kWT +
+                                                kWaves * accfwd::kWaveCap) + wave * kWT;
+  const int lane = threadIdx.x & 63;
   const int32_t b = u.b;
   const uint32_t sb = off[b];
   const uint32_t nl = u.nlow;
@@ -179,7 +288,8 @@ __global__ __launch_bounds__(kWG) void k_fwd_wave(const uint32_t *__restrict__ o
     const uint32_t cb = __builtin_amdgcn_readfirstlane(__shfl(my_cb, k));
     const uint32_t ce = __builtin_amdgcn_readfirstlane(__shfl(my_ce, k));
     uint32_t p = ~0u;
-    int reg = scan_probe_lds(adj, cnt, keys, kpos, ctr, bits, mask, b, cb, ce, lane, &p);
+    const HashSet set{keys, kpos, bits, mask};
+    int reg = scan_probe<U>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, b, cb, ce, lane, &p);
     reg = wave_sum_i32(reg);
     if (lane == 0) {
       cnt[u.j0 + k] = reg;
@@ -193,58 +303,23 @@ __global__ __launch_bounds__(kWG) void k_fwd_wave(const uint32_t *__restrict__ o
   }
 }
 
-// Workgroup units: one chunk of at most kChunk upper slots of a source whose
-// N<(b) fits the workgroup table.
-__global__ __launch_bounds__(kWG) void k_fwd_block(const uint32_t *__restrict__ off,
+// The forward counting launch: workgroups [0, n_block) take one workgroup
+// unit each, the following workgroups take kWaves wave units each.
+template <int U>
+__global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
                                                    const int32_t *__restrict__ adj,
-                                                   const FwdUnit *__restrict__ units, int32_t *cnt,
-                                                   uint32_t *__restrict__ rev) {
-  constexpr int kT = 1 << kTableBits;
-  __shared__ int32_t keys[kT];
-  __shared__ uint16_t kpos[kT];
-  __shared__ int32_t ctr[kLdsCap];
-  __shared__ uint32_t s_cb[kChunk];
-  __shared__ uint32_t s_ce[kChunk];
-  const FwdUnit u = units[blockIdx.x];
-  const int32_t b = u.b;
-  const uint32_t sb = off[b];
-  const uint32_t nl = u.nlow;
-  const uint32_t nup = u.j1 - u.j0;
-  const int tid = threadIdx.x;
-  const int bits = table_bits_for(nl);
-  const uint32_t mask = (1u << bits) - 1;
-  for (uint32_t i = tid; i <= mask; i += kWG) keys[i] = -1;
-  for (uint32_t i = tid; i < nl; i += kWG) ctr[i] = 0;
-  for (uint32_t i = tid; i < nup; i += kWG) {
-    const int32_t c = adj[u.j0 + i];
-    s_cb[i] = off[c];
-    s_ce[i] = off[c + 1];
+                                                   const FwdUnit *__restrict__ block_units, uint32_t n_block,
+                                                   const FwdUnit *__restrict__ wave_units, uint32_t n_wave,
+                                                   int32_t *cnt, uint32_t *__restrict__ rev) {
+  __shared__ __align__(16) unsigned char smem[kCountLdsBytes];
+  if (blockIdx.x < n_block) {
+    fwd_block_unit<U>(off, adj, block_units[blockIdx.x], cnt, rev, smem);
+    return;
   }
-  __syncthreads();
-  lds_insert(keys, kpos, bits, mask, adj, sb, nl, tid, kWG);
-  __syncthreads();
-  // Static round-robin of neighbors over waves.  The loop index must stay
-  // wave-uniform and must not be produced by a lane-0-only branch: a work
-  // counter taken by lane 0 and broadcast lets the compiler rotate the loop so
-  // that lanes 1..63 re-enter it without lane 0 and never exit.
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lane = tid & 63;
-  for (uint32_t k = wave; k < nup; k += kWaves) {
-    uint32_t p = ~0u;
-    const uint32_t cb = __builtin_amdgcn_readfirstlane(s_cb[k]);
-    const uint32_t ce = __builtin_amdgcn_readfirstlane(s_ce[k]);
-    int reg = scan_probe_lds(adj, cnt, keys, kpos, ctr, bits, mask, b, cb, ce, lane, &p);
-    reg = wave_sum_i32(reg);
-    if (lane == 0) {
-      cnt[u.j0 + k] = reg;
-      rev[u.j0 + k] = p;
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = tid; i < nl; i += kWG) {
-    const int v = ctr[i];
-    if (v) atomicAdd(&cnt[sb + i], v);
-  }
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t ui = (blockIdx.x - n_block) * kWaves + wave;
+  if (ui >= n_wave) return;
+  fwd_wave_unit<U>(off, adj, wave_units[ui], cnt, rev, smem, wave);
 }
 
 // Sources whose N<(b) exceeds the workgroup table: membership by binary
@@ -379,20 +454,31 @@ int launch_init(acc_graph *g) {
 
 int launch_count(acc_graph *g) {
   hipStream_t st = g->ctx->stream;
-  if (g->n_fwd_block) {
-    hipLaunchKernelGGL(k_fwd_block, dim3(g->n_fwd_block), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
-                       g->d_cnt, g->d_rev);
+  const unsigned grid = g->n_fwd_block + (g->n_fwd_wave + kWaves - 1) / kWaves;
+  if (grid) {
+    switch (g->fwd_unroll) {
+      case 1:
+        hipLaunchKernelGGL(k_fwd_count<1>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
+                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        break;
+      case 4:
+        hipLaunchKernelGGL(k_fwd_count<4>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
+                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        break;
+      default:
+        hipLaunchKernelGGL(k_fwd_count<2>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
+                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        break;
+    }
     ACC_HIP_TRY(hipGetLastError());
   }
-  if (g->n_fwd_wave) {
-    const unsigned grid = (g->n_fwd_wave + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(k_fwd_wave, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_wave,
-                       g->n_fwd_wave, g->d_cnt, g->d_rev);
-    ACC_HIP_TRY(hipGetLastError());
-  }
+  return ACC_OK;
+}
+
+int launch_count_big(acc_graph *g) {
   if (g->n_fwd_big) {
-    hipLaunchKernelGGL(k_fwd_big, dim3(g->n_fwd_big), dim3(kBigWG), 0, st, g->d_off, g->d_adj, g->d_fwd_big,
-                       g->d_cnt, g->d_rev);
+    hipLaunchKernelGGL(k_fwd_big, dim3(g->n_fwd_big), dim3(kBigWG), 0, g->ctx->stream, g->d_off, g->d_adj,
+                       g->d_fwd_big, g->d_cnt, g->d_rev);
     ACC_HIP_TRY(hipGetLastError());
   }
   return ACC_OK;

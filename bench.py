@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--workload", default="rmat22", choices=sorted(WORKLOADS))
     ap.add_argument("--path", default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--param", action="append", default=[], metavar="NAME=VALUE",
+                    help="tuning parameter of the resident graph (repeatable)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
@@ -129,6 +131,11 @@ def main():
     t0 = time.perf_counter()
     rg = ctx.upload(g)
     t_up = time.perf_counter() - t0
+    params = {}
+    for kv in args.param:
+        k, v = kv.split("=", 1)
+        rg.set_param(k, int(v))
+        params[k] = int(v)
     for _ in range(args.warmup):
         rg.compute(args.path)
     ctx.synchronize()
@@ -172,7 +179,7 @@ def main():
         "config": {"workload": f"{args.workload}: R-MAT scale 22, edge factor 16, (a,b,c,d)=(0.57,0.19,0.19,0.05), "
                                "symmetrised, deduplicated, degree-descending relabel"
                    if args.workload == "rmat22" else args.workload,
-                   "n": g.n, "m": g.m, "path": args.path, "triangles": tri,
+                   "n": g.n, "m": g.m, "path": args.path, "params": params, "triangles": tri,
                    "undirected_edges_per_s": value / 2, "kernel_ms": per_kernel,
                    "b_alg_bytes": b_alg, "counts_symmetric_and_sum_6T": consistent},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -90,6 +90,11 @@ const int32_t *acc_cn_device_counts(const acc_graph *g);
  * (names_cap bytes).  Returns the number of kernels. */
 int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap);
 
+/* Tuning parameters of a resident graph (the knobs the autotuner sets). */
+#define ACC_PARAM_FWD_UNROLL 1 /* 64-element chunks per scan round in the forward path: 1, 2 or 4 */
+int acc_graph_set_param(acc_graph *g, int param, int64_t value);
+int acc_graph_get_param(const acc_graph *g, int param, int64_t *value);
+
 /* Triangle count (SCAN_XP::TriCnt / TriCntSortedArray,
  * scan-query/scan_xp_triangle_cnt.cpp:778-957): runs the counting path and
  * returns T. */

@@ -113,12 +113,25 @@ def test_edge_cases(ctx, path):
     g = G.from_edges(10, [2, 3, 2], [3, 7, 7])
     cnt, t = gpu_counts(ctx, g, path)
     assert (cnt == 3).all() and t == 1
-    # a hub whose forward list exceeds the LDS table: K_{1}+clique of 2600 forces |N<(b)| > 2048
-    k = 2600
-    s, d = np.triu_indices(k, 1)
-    g = G.from_edges(k, s, d)
-    cnt, t = gpu_counts(ctx, g, path)
-    assert (cnt == k).all()
+    # forward lists longer than the LDS hash table: K_2600 on ids below 2^17 (LDS bitmap
+    # with counters spilling to atomics) and K_2200 on ids above 2^17 (global-memory lookups)
+    for k, base in ((2600, 0), (2200, 140_000)):
+        s, d = np.triu_indices(k, 1)
+        g = G.from_edges(base + k, s + base, d + base)
+        cnt, t = gpu_counts(ctx, g, path)
+        assert (cnt == k).all()
+        assert t == k * (k - 1) * (k - 2) // 6
+
+
+@pytest.mark.parametrize("unroll", [1, 2, 4])
+def test_unroll_variants(ctx, unroll):
+    for g in (G.gen_rmat(15, 16, seed=21), G.gen_chung_lu(1 << 14, 2.1, 24, 4096, seed=22)):
+        with ctx.upload(g) as rg:
+            rg.set_param("fwd_unroll", unroll)
+            assert rg.get_param("fwd_unroll") == unroll
+            cnt, t = rg.count_all_edges("forward")
+        assert np.array_equal(cnt, oracle.cn_merge(g))
+        assert t == oracle.triangle_count(g)
 
 
 def test_raw_flag(ctx):
