@@ -28,6 +28,14 @@ SIGNATURES = {
     "acc_triangle_total": (C.c_int, [vp, C.c_int, u64p]),
     "acc_graph_set_param": (C.c_int, [vp, C.c_int, C.c_int64]),
     "acc_graph_get_param": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64)]),
+    "acc_graph_set_vertex_ids": (C.c_int, [vp, i32p]),
+    "acc_scan_ws_create": (C.c_int, [vp, C.POINTER(vp)]),
+    "acc_scan_ws_destroy": (None, [vp]),
+    "acc_scan_ws_info": (C.c_int, [vp, u64p, C.POINTER(vp)]),
+    "acc_scan_run": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_int32, C.c_uint32]),
+    "acc_scan_synchronize": (C.c_int, [vp]),
+    "acc_scan_results": (C.c_int, [vp, i32p, i32p, C.POINTER(C.c_uint8), i32p, C.POINTER(C.c_int64)]),
+    "acc_scan_pairs": (C.c_int, [vp, i32p, C.c_uint64, u64p]),
 }
 
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import weakref
+from fractions import Fraction
 
 import numpy as np
 
@@ -106,6 +107,7 @@ class ResidentGraph:
         check_cn(self._lib.acc_graph_upload(ctx.handle, self.n, self.m, _ffi.ptr(off, C.c_uint32),
                                             _ffi.ptr(adj, C.c_int32), flags, C.byref(h)))
         self._h = h
+        self._workspaces = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -162,7 +164,24 @@ class ResidentGraph:
     def device_counts_ptr(self) -> int:
         return self._lib.acc_cn_device_counts(self._h) or 0
 
+    def set_vertex_ids(self, old_id) -> None:
+        """Report cluster ids and memberships in these ids (old_id[new vertex])."""
+        if old_id is None:
+            check_cn(self._lib.acc_graph_set_vertex_ids(self._h, None))
+            return
+        a = np.ascontiguousarray(old_id, np.int32)
+        if a.shape[0] != self.n:
+            raise ValueError("old_id must have n entries")
+        check_cn(self._lib.acc_graph_set_vertex_ids(self._h, _ffi.ptr(a, C.c_int32)))
+
+    def scan_workspace(self) -> "ScanWorkspace":
+        ws = ScanWorkspace(self)
+        self._workspaces.add(ws)
+        return ws
+
     def release(self) -> None:
+        for ws in list(getattr(self, "_workspaces", ())):
+            ws.close()
         if self._h:
             self._lib.acc_graph_release(self._h)
             self._h = None
@@ -178,6 +197,90 @@ class ResidentGraph:
             self.release()
         except Exception:
             pass
+
+
+class ScanWorkspace:
+    """Buffers and a stream for SCAN queries on one resident graph (``acc_scan_ws``)."""
+
+    def __init__(self, rg: ResidentGraph):
+        self.rg = rg
+        self._lib = rg._lib
+        h = C.c_void_p()
+        check_cn(self._lib.acc_scan_ws_create(rg.handle, C.byref(h)))
+        self._h = h
+
+    def run(self, eps, mu: int) -> None:
+        """Enqueue one query; eps is a decimal string or a float (its shortest repr)."""
+        fr = Fraction(eps) if isinstance(eps, str) else Fraction(repr(float(eps)))
+        check_cn(self._lib.acc_scan_run(self._h, fr.numerator, fr.denominator, int(mu), 0))
+
+    def synchronize(self) -> None:
+        check_cn(self._lib.acc_scan_synchronize(self._h))
+
+    def results(self, sim: bool = False, core_count: bool = False) -> dict:
+        n, m = self.rg.n, self.rg.m
+        label = np.empty(n, np.int32)
+        cid = np.empty(n, np.int32)
+        s = np.empty(m, np.uint8) if sim else None
+        cc = np.empty(n, np.int32) if core_count else None
+        counts = (C.c_int64 * 4)()
+        check_cn(self._lib.acc_scan_results(self._h, _ffi.ptr(label, C.c_int32), _ffi.ptr(cid, C.c_int32),
+                                            _ffi.ptr(s, C.c_uint8) if sim else None,
+                                            _ffi.ptr(cc, C.c_int32) if core_count else None, counts))
+        out = {"label": label, "cluster_id": cid, "core_num": counts[0], "cluster_num": counts[1],
+               "hub_num": counts[2], "outlier_num": counts[3]}
+        if sim:
+            out["sim"] = s
+        if core_count:
+            out["core_count"] = cc
+        return out
+
+    def pairs(self) -> np.ndarray:
+        """Non-core memberships as an int32 array of (cluster id, vertex id) rows, unordered."""
+        k = C.c_uint64(0)
+        check_cn(self._lib.acc_scan_pairs(self._h, None, 0, C.byref(k)))
+        buf = np.empty((max(k.value, 1), 2), np.int32)
+        check_cn(self._lib.acc_scan_pairs(self._h, _ffi.ptr(buf, C.c_int32), k.value, C.byref(k)))
+        return buf[:k.value]
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.acc_scan_ws_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_scan_result(path: str, n: int, label, cluster_id, pairs, old_id=None) -> None:
+    """The reference's result file (Graph::Output with old_vid_dict)."""
+    from ._ffi import check_graph, graph_lib
+
+    lab = np.ascontiguousarray(label, np.int32)
+    cid = np.ascontiguousarray(cluster_id, np.int32)
+    pr = np.ascontiguousarray(pairs, np.int32).copy()
+    oid = None if old_id is None else np.ascontiguousarray(old_id, np.int32)
+    check_graph(graph_lib().acc_write_scan_result(path.encode(), n, _ffi.ptr(lab, C.c_int32),
+                                                  _ffi.ptr(cid, C.c_int32),
+                                                  None if oid is None else _ffi.ptr(oid, C.c_int32),
+                                                  _ffi.ptr(pr, C.c_int32), pr.shape[0]))
+
+
+def scan_result_path(directory: str, eps: float, mu: int) -> str:
+    from ._ffi import check_graph, graph_lib
+
+    buf = C.create_string_buffer(4096)
+    check_graph(graph_lib().acc_scan_result_path(directory.encode(), float(eps), int(mu), buf, 4096))
+    return buf.value.decode()
 
 
 __all__ = ["Context", "ResidentGraph", "AccError", "device_count", "PATH_AUTO", "PATH_FORWARD",

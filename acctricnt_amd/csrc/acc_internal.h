@@ -53,6 +53,9 @@ struct acc_graph {
   uint32_t *d_rev = nullptr;            // scratch: reverse slot of every upper slot, ~0 elsewhere
   unsigned long long *d_sum = nullptr;  // sum over upper slots of the raw counts (= 3T)
   int32_t *d_flags = nullptr;           // validation flags
+  int32_t *d_src = nullptr;             // source vertex of every slot (built for SCAN)
+  int32_t *d_old = nullptr;             // caller vertex ids for cluster ids (NULL: identity)
+  bool cnt_plus2 = true;                // counts stored as 2 + |N(u) ∩ N(v)|
   // forward plan, binned by degree skew
   FwdUnit *d_fwd_wave = nullptr;  // few upper slots, small N<(b): one wave per source
   uint32_t n_fwd_wave = 0;
@@ -62,13 +65,16 @@ struct acc_graph {
   uint32_t n_fwd_big = 0;
   uint64_t device_bytes = 0;
   // tuning parameters (acc_graph_set_param)
-  int fwd_unroll = 2;
+  int fwd_unroll = 4;
   // last results / timing
   bool have_result = false;
   std::vector<hipEvent_t> events;
   std::vector<std::string> event_names;
   bool timing_valid = false;
 };
+
+// Builds the slot -> source array (acc_scan.hip).
+int acc_graph_ensure_src(acc_graph *g);
 
 // Kernel launchers (k_forward.hip).
 namespace accfwd {

@@ -33,7 +33,8 @@ void free_graph(acc_graph *g) {
   for (void *p : {static_cast<void *>(g->d_off), static_cast<void *>(g->d_adj), static_cast<void *>(g->d_cnt),
                   static_cast<void *>(g->d_rev), static_cast<void *>(g->d_sum), static_cast<void *>(g->d_flags),
                   static_cast<void *>(g->d_fwd_wave), static_cast<void *>(g->d_fwd_block),
-                  static_cast<void *>(g->d_fwd_big)})
+                  static_cast<void *>(g->d_fwd_big), static_cast<void *>(g->d_src),
+                  static_cast<void *>(g->d_old)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
   delete g;
@@ -243,6 +244,7 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   ACC_TRY(accfwd::launch_combine(g, (flags & ACC_CN_RAW) ? 0 : 2));
   ACC_TRY(record(g, timing, "k_combine"));
   g->have_result = true;
+  g->cnt_plus2 = (flags & ACC_CN_RAW) == 0;
   g->timing_valid = timing;
   return ACC_OK;
 }

@@ -465,6 +465,53 @@ int acc_convert_edge_list(const char *txt_path, const char *out_dir) {
   return rc;
 }
 
+int acc_write_scan_result(const char *path, uint32_t n, const int32_t *label, const int32_t *cluster_id,
+                          const int32_t *old_id, int32_t *pairs, uint64_t npairs) {
+  if (!path || (n && (!label || !cluster_id)) || (npairs && !pairs)) return fail(ACC_EINVAL, "bad argument");
+  std::vector<std::pair<int32_t, int32_t>> cores;
+  for (uint32_t u = 0; u < n; u++)
+    if (label[u] == 1) cores.emplace_back(old_id ? old_id[u] : static_cast<int32_t>(u), cluster_id[u]);
+  std::sort(cores.begin(), cores.end());
+  auto *pp = reinterpret_cast<std::pair<int32_t, int32_t> *>(pairs);
+  static_assert(sizeof(std::pair<int32_t, int32_t>) == 8, "pair layout");
+  std::sort(pp, pp + npairs);
+  FILE *f = std::fopen(path, "w");
+  if (!f) return fail(ACC_EIO, std::string("cannot write ") + path);
+  std::string buf;
+  buf.reserve(1 << 20);
+  buf += "c/n vertex_id cluster_id\n";
+  char line[64];
+  bool ok = true;
+  auto flush = [&]() {
+    if (!buf.empty() && std::fwrite(buf.data(), 1, buf.size(), f) != buf.size()) ok = false;
+    buf.clear();
+  };
+  for (auto &c : cores) {
+    std::snprintf(line, sizeof(line), "c %d %d\n", c.first, c.second);
+    buf += line;
+    if (buf.size() > (1 << 20)) flush();
+  }
+  for (uint64_t i = 0; i < npairs; i++) {
+    if (i > 0 && pp[i] == pp[i - 1]) continue;
+    std::snprintf(line, sizeof(line), "n %d %d\n", pp[i].second, pp[i].first);
+    buf += line;
+    if (buf.size() > (1 << 20)) flush();
+  }
+  flush();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? ACC_OK : fail(ACC_EIO, std::string("short write to ") + path);
+}
+
+int acc_scan_result_path(const char *dir, double eps, int mu, char *buf, int cap) {
+  if (!dir || !buf || cap <= 0) return fail(ACC_EINVAL, "bad argument");
+  std::string e = std::to_string(eps);
+  e.erase(e.find_last_not_of("0u") + 1);
+  std::string p = std::string(dir) + "/scanxp-result-" + e + "-" + std::to_string(mu) + ".txt";
+  if (static_cast<int>(p.size()) >= cap) return fail(ACC_EINVAL, "buffer too small");
+  std::memcpy(buf, p.c_str(), p.size() + 1);
+  return ACC_OK;
+}
+
 int acc_hgraph_reverse_slots(const acc_hgraph *g, uint32_t *rev) {
   if (!g || !rev) return fail(ACC_EINVAL, "bad argument");
 #pragma omp parallel for schedule(dynamic, 1024)

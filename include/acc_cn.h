@@ -100,6 +100,39 @@ int acc_graph_get_param(const acc_graph *g, int param, int64_t *value);
  * returns T. */
 int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles);
 
+/* SCAN queries ---------------------------------------------------------- */
+/* Structural clustering on the resident counts, replacing the reference's
+ * post-processing after CheckCore: PostComputeCoreChecking / CheckCoreCompSimCore
+ * (scan-query-cuda/scan_xp_common.cu:220-268, scan-query/scan_xp.cpp:112-141),
+ * ClusterCore (:1915-1928), LabelNonCore (:1940-1955), PostProcess
+ * (:1957-1981), MarkClusterMinEleAsId (:1983-2020) and the non-core part of
+ * PrepareResultOutput (:2022-2049).  eps = eps_num / eps_den exactly; a slot is
+ * similar iff eps_den^2 cnt^2 >= eps_num^2 (d_u+1)(d_v+1), the exact form of the
+ * reference's long-double test.  A vertex is core iff it has at least mu similar
+ * neighbors; cluster ids are the minimum vertex id (see acc_graph_set_vertex_ids)
+ * of the cluster's cores. */
+typedef struct acc_scan_ws acc_scan_ws; /* per-query buffers and a stream */
+
+/* Ids reported as cluster ids and in non-core pairs: old_id[new vertex] (the
+ * reference's old_vid_dict after a relabel).  NULL restores the identity. */
+int acc_graph_set_vertex_ids(acc_graph *g, const int32_t *old_id);
+int acc_scan_ws_create(acc_graph *g, acc_scan_ws **out);
+void acc_scan_ws_destroy(acc_scan_ws *ws);
+/* Device bytes held and the workspace's HIP stream (a hipStream_t). */
+int acc_scan_ws_info(const acc_scan_ws *ws, uint64_t *device_bytes, void **stream);
+/* Enqueue one query on the workspace stream, ordered after the graph's counts
+ * (computed first with ACC_PATH_AUTO if the graph has none). */
+int acc_scan_run(acc_scan_ws *ws, int64_t eps_num, int64_t eps_den, int32_t mu, uint32_t flags);
+int acc_scan_synchronize(acc_scan_ws *ws);
+/* Copy results: label[n] (0 outlier, 1 core, 2 hub; the reference's
+ * UNCLASSIFIED/CORE/HUB), cluster_id[n] (-1 for non-cores), sim[m] (0/1),
+ * core_count[n], counts[4] = {cores, clusters, hubs, outliers}.  Any may be NULL. */
+int acc_scan_results(acc_scan_ws *ws, int32_t *label, int32_t *cluster_id, uint8_t *sim, int32_t *core_count,
+                     int64_t *counts);
+/* Non-core memberships (cluster id, vertex id) as int32 pairs in pairs[2*cap],
+ * unordered; *count receives the total (which may exceed cap). */
+int acc_scan_pairs(acc_scan_ws *ws, int32_t *pairs, uint64_t cap, uint64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,20 @@ int acc_hgraph_read_bins(const char *dir, acc_hgraph **out);
  * '#' lines skipped, only lines with first < second kept, n = max id + 1. */
 int acc_convert_edge_list(const char *txt_path, const char *out_dir);
 
+/* SCAN result file of the reference (Graph::Output(..., old_vid_dict),
+ * scan-query/util/graph.cpp:176-216; name built by SCAN_XP::PrepareResultOutput,
+ * scan-query/scan_xp.cpp:2056-2069): "c/n vertex_id cluster_id", then
+ * "c <id> <cluster id>" for every core (label 1) in ascending id order, then
+ * "n <id> <cluster id>" for the sorted unique (cluster id, id) pairs.  ids are
+ * old_id[v] when old_id is not NULL.  pairs holds npairs (cluster id, id)
+ * int32 pairs in any order, duplicates allowed; it is sorted in place. */
+int acc_write_scan_result(const char *path, uint32_t n, const int32_t *label, const int32_t *cluster_id,
+                          const int32_t *old_id, int32_t *pairs, uint64_t npairs);
+
+/* "<dir>/scanxp-result-<eps>-<mu>.txt" with eps formatted as to_string(eps)
+ * with trailing '0' and 'u' characters removed (scan-query/scan_xp.cpp:2060-2061). */
+int acc_scan_result_path(const char *dir, double eps, int mu, char *buf, int cap);
+
 /* Reverse-slot index on the host: rev[i] = slot of (v,u) for slot i = (u,v). */
 int acc_hgraph_reverse_slots(const acc_hgraph *g, uint32_t *rev_out);
 

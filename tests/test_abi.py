@@ -134,3 +134,33 @@ def test_algorithmic_bytes_formula():
     g = BF.complete(5)
     # n=5, m=20, every undirected edge has min degree 4, 10 edges
     assert g.algorithmic_bytes() == 4 * 6 + 8 * 20 + 4 * 10 * 4
+
+
+def test_result_writer_matches_oracle(tmp_path):
+    """The library's result-file writer reproduces the oracle's file byte for byte,
+    with and without an old-id mapping."""
+    import oracle
+    from acctricnt_amd import cn
+
+    for relabel in (False, True):
+        g = G.gen_rmat(10, 12, seed=31, relabel=relabel)
+        old = g.old_of_new if relabel else None
+        res = oracle.scan(g, oracle.cn_merge(g), "0.4", 3, old_id=old)
+        a = tmp_path / f"a{relabel}.txt"
+        oracle.write_result(str(a), g, res, old_id=old)
+        src = g.sources()
+        core = res["label"] == 1
+        sel = res["sim"].astype(bool) & core[src] & ~core[g.edge_dst]
+        ids = g.edge_dst[sel] if old is None else old[g.edge_dst[sel]]
+        pairs = np.stack([res["cluster_id"][src[sel]], ids], axis=1)
+        pairs = np.concatenate([pairs, pairs[:3]])  # duplicates are removed by the writer
+        b = tmp_path / f"b{relabel}.txt"
+        cn.write_scan_result(str(b), g.n, res["label"], res["cluster_id"], pairs, old_id=old)
+        assert a.read_bytes() == b.read_bytes()
+
+
+def test_result_path_format(tmp_path):
+    from acctricnt_amd import cn
+
+    assert cn.scan_result_path("/d", 0.2, 5) == "/d/scanxp-result-0.2-5.txt"
+    assert cn.scan_result_path("/d", 1.0, 2) == "/d/scanxp-result-1.-2.txt"
