@@ -1,0 +1,332 @@
+// Drop-in SCAN_XP / Graph on top of libacc_cn.so.  Step structure, log tags
+// and output follow the reference's SCAN_XP::Execute (scan-query/scan_xp.cpp:2076-2124,
+// scan-query-cuda/scan_xp_common.cu:334-360) and SCAN_XP::TriCnt
+// (scan-query/scan_xp_triangle_cnt.cpp:62-116, 778-957).
+
+#include "scan_xp.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "log.h"
+
+using namespace std::chrono;
+
+namespace {
+
+[[noreturn]] void die(const char *what, const char *detail) {
+  log_error("%s: %s", what, detail);
+  std::exit(1);
+}
+
+void check_cn(int rc, const char *what) {
+  if (rc != ACC_OK) die(what, acc_last_error());
+}
+
+double seconds_since(high_resolution_clock::time_point t0) {
+  return duration_cast<nanoseconds>(high_resolution_clock::now() - t0).count() / 1e9;
+}
+
+// Relabels a CSR by old_of_new (new id -> old id): new lists are the mapped,
+// sorted old lists (SCAN_XP::Reorder, scan-query/scan_xp.cpp:1145-1222).
+void relabel(uint32_t n, const uint32_t *off, const int32_t *adj, const std::vector<int> &old_of_new,
+             std::vector<uint32_t> &noff, std::vector<int32_t> &nadj) {
+  std::vector<int32_t> new_of_old(n);
+  for (uint32_t x = 0; x < n; x++) new_of_old[old_of_new[x]] = static_cast<int32_t>(x);
+  noff.assign(static_cast<size_t>(n) + 1, 0);
+  for (uint32_t x = 0; x < n; x++) {
+    const uint32_t o = old_of_new[x];
+    noff[x + 1] = noff[x] + (off[o + 1] - off[o]);
+  }
+  nadj.resize(noff[n]);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long x = 0; x < static_cast<long long>(n); x++) {
+    const uint32_t o = old_of_new[x];
+    uint32_t w = noff[x];
+    for (uint32_t j = off[o]; j < off[o + 1]; j++) nadj[w++] = new_of_old[adj[j]];
+    std::sort(nadj.begin() + noff[x], nadj.begin() + noff[x + 1]);
+  }
+}
+
+}  // namespace
+
+bool ParseEps(const char *text, int64_t *num, int64_t *den) {
+  std::string s(text ? text : "");
+  if (s.find_first_of("eEnN") != std::string::npos) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%.15f", std::strtod(s.c_str(), nullptr));
+    s = buf;
+    s.erase(s.find_last_not_of('0') + 1);
+  }
+  if (!s.empty() && s[0] == '+') s.erase(0, 1);
+  int64_t n = 0, d = 1;
+  bool dot = false, any = false;
+  for (char ch : s) {
+    if (ch == '.' && !dot) {
+      dot = true;
+      continue;
+    }
+    if (ch < '0' || ch > '9') return false;
+    if (n > (int64_t(1) << 40)) return false;
+    n = n * 10 + (ch - '0');
+    if (dot) d *= 10;
+    any = true;
+  }
+  if (!any) return false;
+  int64_t a = n, b = d;
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  if (a > 1) {
+    n /= a;
+    d /= a;
+  }
+  if (n > (int64_t(1) << 31) || d > (int64_t(1) << 31)) return false;
+  *num = n;
+  *den = d;
+  return true;
+}
+
+// ---------------------------------------------------------------- Graph
+Graph::Graph(const char *dir_cstr) : dir(dir_cstr) { Load(); }
+
+Graph::~Graph() {
+  delete[] node_off;
+  delete[] edge_dst;
+  delete[] similarity;
+  delete[] common_node_num;
+  delete[] core_count;
+  delete[] label;
+}
+
+void Graph::Load() {
+  auto start = high_resolution_clock::now();
+  acc_hgraph *h = nullptr;
+  if (acc_hgraph_read_bins(dir.c_str(), &h) != ACC_OK) {
+    const std::string err = acc_graph_last_error();
+    if (err.find("Self loop") != std::string::npos) std::printf("Self loop\n");
+    die("loading graph", err.c_str());
+  }
+  const uint64_t m = acc_hgraph_m(h);
+  if (m >= (1ULL << 32)) die("loading graph", "m does not fit uint32 offsets");
+  nodemax = acc_hgraph_n(h);
+  edgemax = static_cast<uint32_t>(m);
+  log_info("int size: %d, n: %s, m: %s", 4, FormatWithCommas(nodemax).c_str(), FormatWithCommas(edgemax).c_str());
+  node_off = new uint32_t[nodemax + 1];
+  edge_dst = new int[static_cast<uint64_t>(edgemax) + 16];
+  std::memcpy(node_off, acc_hgraph_offsets(h), sizeof(uint32_t) * (nodemax + 1ULL));
+  std::memcpy(edge_dst, acc_hgraph_adj(h), sizeof(int) * static_cast<uint64_t>(edgemax));
+  acc_hgraph_free(h);
+  degree.resize(nodemax);
+  for (uint32_t i = 0; i < nodemax; i++) degree[i] = static_cast<int>(node_off[i + 1] - node_off[i]) + 1;
+  label = new int[nodemax];
+  core_count = new int[nodemax];
+  common_node_num = new int[edgemax ? edgemax : 1];
+  similarity = new bool[edgemax ? edgemax : 1];
+  log_info("read adjacency list file time: %.3lf s", seconds_since(start));
+}
+
+void Graph::ReplaceCSR(const uint32_t *off, const int32_t *adj) {
+  std::memcpy(node_off, off, sizeof(uint32_t) * (nodemax + 1ULL));
+  std::memcpy(edge_dst, adj, sizeof(int) * static_cast<uint64_t>(edgemax));
+  for (uint32_t i = 0; i < nodemax; i++) degree[i] = static_cast<int>(node_off[i + 1] - node_off[i]) + 1;
+}
+
+void Graph::Output(const char *eps_s, const char *min_u, const std::vector<int> &old_vid_dict) {
+  const std::string out_name = dir + "/scanxp-result-" + std::string(eps_s) + "-" + std::string(min_u) + ".txt";
+  std::vector<int32_t> pairs(2 * noncore_cluster.size());
+  for (size_t i = 0; i < noncore_cluster.size(); i++) {
+    pairs[2 * i] = noncore_cluster[i].first;
+    pairs[2 * i + 1] = noncore_cluster[i].second;
+  }
+  const int32_t *old = old_vid_dict.empty() ? nullptr : old_vid_dict.data();
+  if (acc_write_scan_result(out_name.c_str(), nodemax, label, cluster_dict.data(), old, pairs.data(),
+                            noncore_cluster.size()) != ACC_OK)
+    die("writing result", acc_graph_last_error());
+}
+
+// ---------------------------------------------------------------- SCAN_XP
+SCAN_XP::SCAN_XP(int thread_num, int min_u, double epsilon, char *dir_cstr, const char *eps_text)
+    : thread_num_(thread_num), min_u_(min_u), epsilon_(epsilon), g(dir_cstr) {
+  char buf[64];
+  if (!eps_text) {
+    std::snprintf(buf, sizeof(buf), "%.15g", epsilon);
+    eps_text = buf;
+  }
+  if (!ParseEps(eps_text, &eps_num_, &eps_den_)) die("epsilon", "not a decimal number in [0, 2^31]");
+  log_info("thread num: %d", thread_num_);
+  log_info("graph, n: %s, m: %s", FormatWithCommas(g.nodemax).c_str(), FormatWithCommas(g.edgemax).c_str());
+  // the reference's constructor initialisation (scan-query/scan_xp.cpp:72-81)
+  for (uint32_t i = 0; i < g.nodemax; i++) {
+    g.core_count[i] = 0;
+    g.label[i] = UNCLASSIFIED;
+  }
+  std::fill(g.common_node_num, g.common_node_num + g.edgemax, 2);
+}
+
+SCAN_XP::SCAN_XP(int thread_num, char *dir_cstr) : thread_num_(thread_num), g(dir_cstr) {}
+
+SCAN_XP::~SCAN_XP() {
+  acc_scan_ws_destroy(ws_);
+  acc_graph_release(dg_);
+  acc_ctx_destroy(ctx_);
+}
+
+void SCAN_XP::Reorder() {
+  auto clk_beg = high_resolution_clock::now();
+  old_vid_dict.clear();
+  if (reorder_method == "deg") {
+    log_info("Re-ordering... Method: DODG");
+    acc_hgraph *h = nullptr;
+    if (acc_hgraph_from_csr(g.nodemax, g.edgemax, g.node_off, g.edge_dst, &h) != ACC_OK)
+      die("re-ordering", acc_graph_last_error());
+    old_vid_dict.resize(g.nodemax);
+    if (acc_hgraph_relabel_degree(h, old_vid_dict.data()) != ACC_OK) die("re-ordering", acc_graph_last_error());
+    g.ReplaceCSR(acc_hgraph_offsets(h), acc_hgraph_adj(h));
+    acc_hgraph_free(h);
+  } else if (reorder_method == "gro" || reorder_method == "cache" || reorder_method == "bfsr" ||
+             reorder_method == "dfs" || reorder_method == "hybrid" || reorder_method == "rcm-cache" ||
+             reorder_method == "slashburn") {
+    // <dir>/<method>.dict: size_t length + int32 new id of every old id
+    // (YcheSerializer::read_array, scan-query/util/yche_serialization.h:72-80)
+    log_info("Re-ordering... Method: %s", reorder_method.c_str());
+    const std::string path = g.dir + "/" + reorder_method + ".dict";
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) die("re-ordering", ("cannot open " + path).c_str());
+    size_t len = 0;
+    std::vector<int> new_vid_dict;
+    if (std::fread(&len, sizeof(size_t), 1, f) == 1 && len == g.nodemax) {
+      new_vid_dict.resize(len);
+      if (std::fread(new_vid_dict.data(), sizeof(int), len, f) != len) new_vid_dict.clear();
+    }
+    std::fclose(f);
+    if (new_vid_dict.size() != g.nodemax) die("re-ordering", "dictionary size differs from n");
+    old_vid_dict.assign(g.nodemax, -1);
+    for (uint32_t v = 0; v < g.nodemax; v++) {
+      const int nv = new_vid_dict[v];
+      if (nv < 0 || static_cast<uint32_t>(nv) >= g.nodemax || old_vid_dict[nv] != -1)
+        die("re-ordering", "dictionary is not a permutation");
+      old_vid_dict[nv] = static_cast<int>(v);
+    }
+    log_info("Finish loading %s", path.c_str());
+    std::vector<uint32_t> noff;
+    std::vector<int32_t> nadj;
+    relabel(g.nodemax, g.node_off, g.edge_dst, old_vid_dict, noff, nadj);
+    g.ReplaceCSR(noff.data(), nadj.data());
+  } else {
+    if (reorder_method == "kcore" || reorder_method == "random")
+      log_warn("Re-ordering method %s is not part of this build; counts and clusters do not depend on it",
+               reorder_method.c_str());
+    log_info("Preserving the original order...");
+  }
+  log_info("PreProcess Reordering Cost: %.9lf s, %s KB", seconds_since(clk_beg), FormatWithCommas(getValue()).c_str());
+}
+
+void SCAN_XP::Upload() {
+  auto t0 = high_resolution_clock::now();
+  check_cn(acc_ctx_create(0, &ctx_), "device");
+  check_cn(acc_graph_upload(ctx_, g.nodemax, g.edgemax, g.node_off, g.edge_dst, 0, &dg_), "graph upload");
+  if (!old_vid_dict.empty()) check_cn(acc_graph_set_vertex_ids(dg_, old_vid_dict.data()), "vertex ids");
+  log_info("GPU upload and validation cost: %.3lf s", seconds_since(t0));
+}
+
+void SCAN_XP::CheckCore(Graph *gp) {
+  Reorder();
+  Upload();
+  auto start = high_resolution_clock::now();
+  check_cn(acc_cn_compute(dg_, ACC_PATH_AUTO, 0), "all-edge count");
+  check_cn(acc_ctx_synchronize(ctx_), "all-edge count");
+  log_info("All-Edge Computation Cost (GPU): %.9lf s, %s KB", seconds_since(start),
+           FormatWithCommas(getValue()).c_str());
+  check_cn(acc_cn_download(dg_, gp->common_node_num, nullptr), "count download");
+  log_debug("All-Edge CN-Cnt Cost: %.3lf s, Mem Usage: %s KB", seconds_since(start),
+            FormatWithCommas(getValue()).c_str());
+
+  // similarity + core check (CheckCoreCompSimCore)
+  auto s2 = high_resolution_clock::now();
+  check_cn(acc_scan_ws_create(dg_, &ws_), "SCAN workspace");
+  check_cn(acc_scan_run(ws_, eps_num_, eps_den_, min_u_, 0), "SCAN query");
+  std::vector<uint8_t> sim(gp->edgemax);
+  check_cn(acc_scan_results(ws_, nullptr, nullptr, sim.data(), gp->core_count, counts_), "SCAN results");
+  for (uint32_t i = 0; i < gp->edgemax; i++) gp->similarity[i] = sim[i] != 0;
+  for (uint32_t u = 0; u < gp->nodemax; u++) gp->label[u] = gp->core_count[u] >= min_u_ ? CORE : UNCLASSIFIED;
+  log_info("core-checking sim-core-comp cost : %.3lf s, Mem Usage: %s KB", seconds_since(s2),
+           FormatWithCommas(getValue()).c_str());
+}
+
+void SCAN_XP::ClusterCore() {
+  std::vector<int32_t> cid(g.nodemax);
+  check_cn(acc_scan_results(ws_, nullptr, cid.data(), nullptr, nullptr, counts_), "cluster ids");
+  g.cluster_dict.assign(cid.begin(), cid.end());
+}
+
+void SCAN_XP::LabelNonCore() {
+  check_cn(acc_scan_results(ws_, g.label, nullptr, nullptr, nullptr, counts_), "labels");
+  log_info("Core: %s", FormatWithCommas(counts_[0]).c_str());
+}
+
+void SCAN_XP::PostProcess() {
+  log_info("Cluster: %s, Hub: %s, Outlier: %s", FormatWithCommas(counts_[1]).c_str(),
+           FormatWithCommas(counts_[2]).c_str(), FormatWithCommas(counts_[3]).c_str());
+}
+
+void SCAN_XP::PrepareResultOutput() {
+  auto start = high_resolution_clock::now();
+  log_info("Step4 - cluster id initialization cost: %.3lf s, Mem Usage: %s KB", seconds_since(start),
+           FormatWithCommas(getValue()).c_str());
+  start = high_resolution_clock::now();
+  uint64_t k = 0;
+  check_cn(acc_scan_pairs(ws_, nullptr, 0, &k), "non-core memberships");
+  std::vector<int32_t> pairs(2 * (k ? k : 1));
+  check_cn(acc_scan_pairs(ws_, pairs.data(), k, &k), "non-core memberships");
+  g.noncore_cluster.resize(k);
+  for (uint64_t i = 0; i < k; i++) g.noncore_cluster[i] = {pairs[2 * i], pairs[2 * i + 1]};
+  log_info("Step4 - prepare results: %.3lf s, Mem Usage: %s KB", seconds_since(start),
+           FormatWithCommas(getValue()).c_str());
+
+  auto epsilon_str = std::to_string(epsilon_);
+  epsilon_str.erase(epsilon_str.find_last_not_of("0u") + 1);
+  start = high_resolution_clock::now();
+  g.Output(epsilon_str.c_str(), std::to_string(min_u_).c_str(), old_vid_dict);
+  log_info("Step4 - output to the disk cost: %.3lf s, Mem Usage: %s KB", seconds_since(start),
+           FormatWithCommas(getValue()).c_str());
+}
+
+void SCAN_XP::Execute() {
+  double s1 = omp_get_wtime();
+  CheckCore(&g);
+  double e1 = omp_get_wtime();
+  log_info("Step1 - CheckCore: %.3lf s, Mem Usage: %s KB", e1 - s1, FormatWithCommas(getValue()).c_str());
+
+  double s2 = omp_get_wtime();
+  ClusterCore();
+  double e2 = omp_get_wtime();
+  log_info("Step2 - ClusterCore: %.3lf s, Mem Usage: %s KB", e2 - s2, FormatWithCommas(getValue()).c_str());
+
+  double s3 = omp_get_wtime();
+  LabelNonCore();
+  double e3 = omp_get_wtime();
+  log_info("Step3 - LabelNonCore: %.3lf s, Mem Usage: %s KB", e3 - s3, FormatWithCommas(getValue()).c_str());
+
+  PostProcess();
+  PrepareResultOutput();
+}
+
+void SCAN_XP::TriCnt() {
+  Reorder();
+  Upload();
+  auto start = high_resolution_clock::now();
+  uint64_t triangles = 0;
+  check_cn(acc_triangle_total(dg_, ACC_PATH_AUTO, &triangles), "triangle count");
+  log_info("Forward cost: %.3lf s, Mem Usage: %s KB", seconds_since(start), FormatWithCommas(getValue()).c_str());
+  log_info("Triangle Cnt: %s", FormatWithCommas(static_cast<long long>(triangles)).c_str());
+}

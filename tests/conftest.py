@@ -21,7 +21,8 @@ def built():
     oracle.lib()
     from acctricnt_amd import _ffi
 
-    if not (os.path.exists(_ffi.GRAPH_LIB_PATH) and os.path.exists(_ffi.CN_LIB_PATH)):
+    bins = [os.path.join(ROOT, "acctricnt_amd", "bin", b) for b in ("main_scanxp", "main_triangle_count")]
+    if not all(os.path.exists(p) for p in [_ffi.GRAPH_LIB_PATH, _ffi.CN_LIB_PATH, *bins]):
         import subprocess
 
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "acctricnt_amd", "csrc")], check=True)
