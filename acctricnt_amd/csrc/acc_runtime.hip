@@ -236,7 +236,7 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   ACC_TRY(accfwd::launch_init(g));
   ACC_TRY(record(g, timing, "k_init"));
   ACC_TRY(accfwd::launch_count(g));
-  ACC_TRY(record(g, timing, "k_fwd_count"));
+  ACC_TRY(record(g, timing, ("k_fwd_count<" + std::to_string(g->fwd_unroll) + ">").c_str()));
   if (g->n_fwd_big) {
     ACC_TRY(accfwd::launch_count_big(g));
     ACC_TRY(record(g, timing, "k_fwd_big"));

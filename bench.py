@@ -72,6 +72,21 @@ def cpu_baseline(g, target_s: float):
                       f"merge-counted by the oracle in {t:.2f} s; full-graph estimate {t * up.size / k:.1f} s"}
 
 
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    summaries in profiles/ (FETCH_SIZE + WRITE_SIZE, calibrated against a plain
+    copy), or None when no summary covers this kernel and workload."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel \
+                    and parts[2] == workload:
+                return int(parts[3])
+    return None
+
+
 def calibration_copy(mib: int) -> None:
     """One device-to-device copy of mib MiB through the HIP runtime: the plain
     copy dispatch whose FETCH_SIZE / WRITE_SIZE counter readings scale the
@@ -183,7 +198,8 @@ def main():
                    "undirected_edges_per_s": value / 2, "kernel_ms": per_kernel,
                    "b_alg_bytes": b_alg, "counts_symmetric_and_sum_6T": consistent},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, args.workload),
+                     "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
