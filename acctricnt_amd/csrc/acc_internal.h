@@ -63,15 +63,32 @@ struct acc_graph {
   uint32_t n_fwd_block = 0;
   FwdUnit *d_fwd_big = nullptr;  // |N<(b)| beyond the LDS table: global-memory lookups
   uint32_t n_fwd_big = 0;
+  // per-edge plans (bitmap and merge paths): chunks of upper slots of every source
+  FwdUnit *d_edge_units = nullptr;  // all sources (merge path)
+  uint32_t n_edge_units = 0;
+  FwdUnit *d_bmp_lds = nullptr;  // sources with d_u <= kLdsCap (bitmap path, LDS set)
+  uint32_t n_bmp_lds = 0;
+  FwdUnit *d_bmp_hub = nullptr;  // hub sources (bitmap path, HBM bitmap)
+  uint32_t n_bmp_hub = 0;
+  uint32_t *d_bmp_pool = nullptr;  // one zeroed n-bit bitmap per persistent workgroup
+  uint32_t bmp_pool_slots = 0;
+  uint64_t bmp_pool_words = 0;
   uint64_t device_bytes = 0;
   // tuning parameters (acc_graph_set_param)
   int fwd_unroll = 4;
+  int auto_path = ACC_PATH_FORWARD;  // what ACC_PATH_AUTO runs
   // last results / timing
   bool have_result = false;
   std::vector<hipEvent_t> events;
   std::vector<std::string> event_names;
   bool timing_valid = false;
 };
+
+// Per-edge path launchers (k_paths.hip).
+namespace accpaths {
+int launch_bitmap(acc_graph *g, int plus);
+int launch_merge(acc_graph *g, int plus);
+}  // namespace accpaths
 
 // Builds the slot -> source array (acc_scan.hip).
 int acc_graph_ensure_src(acc_graph *g);

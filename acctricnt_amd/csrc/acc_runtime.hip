@@ -34,7 +34,9 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_rev), static_cast<void *>(g->d_sum), static_cast<void *>(g->d_flags),
                   static_cast<void *>(g->d_fwd_wave), static_cast<void *>(g->d_fwd_block),
                   static_cast<void *>(g->d_fwd_big), static_cast<void *>(g->d_src),
-                  static_cast<void *>(g->d_old)})
+                  static_cast<void *>(g->d_old), static_cast<void *>(g->d_edge_units),
+                  static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
+                  static_cast<void *>(g->d_bmp_pool)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
   delete g;
@@ -68,6 +70,33 @@ void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std
       (lds ? block : big).push_back(u);
     }
   }
+}
+
+// Per-edge plan: chunks of at most kChunk upper slots of every source (merge
+// path); the bitmap path splits them by d_u into LDS-set and hub units.
+void build_edge_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &all,
+                     std::vector<FwdUnit> &lds, std::vector<FwdUnit> &hub) {
+  for (uint32_t u = 0; u < n; u++) {
+    const int32_t *ub = std::upper_bound(adj + off[u], adj + off[u + 1], static_cast<int32_t>(u));
+    const uint32_t hs = static_cast<uint32_t>(ub - adj);
+    const uint32_t du = off[u + 1] - off[u];
+    for (uint32_t j0 = hs; j0 < off[u + 1]; j0 += accfwd::kChunk) {
+      FwdUnit un{static_cast<int32_t>(u), j0, std::min(j0 + accfwd::kChunk, off[u + 1]), du};
+      all.push_back(un);
+      (du <= accfwd::kLdsCap ? lds : hub).push_back(un);
+    }
+  }
+}
+
+int ensure_bitmap_pool(acc_graph *g) {
+  if (g->d_bmp_pool || !g->n_bmp_hub) return ACC_OK;
+  const uint32_t slots = static_cast<uint32_t>(std::max(1, g->ctx->num_cus) * 2);
+  const uint64_t words = (static_cast<uint64_t>(g->n) + 31) / 32 + 1;
+  ACC_TRY(dev_alloc(g, &g->d_bmp_pool, words * slots));
+  ACC_HIP_TRY(hipMemsetAsync(g->d_bmp_pool, 0, sizeof(uint32_t) * words * slots, g->ctx->stream));
+  g->bmp_pool_slots = slots;
+  g->bmp_pool_words = words;
+  return ACC_OK;
 }
 
 int record(acc_graph *g, bool timing, const char *name) {
@@ -191,14 +220,17 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
     if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
-  std::vector<FwdUnit> wave, block, big;
+  std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
   build_forward_plan(n, node_off, edge_dst, wave, block, big, g->max_low, g->max_deg);
-  struct {
+  build_edge_plan(n, node_off, edge_dst, eall, elds, ehub);
+  struct PlanList {
     std::vector<FwdUnit> *v;
     FwdUnit **d;
     uint32_t *count;
-  } lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave}, {&block, &g->d_fwd_block, &g->n_fwd_block},
-               {&big, &g->d_fwd_big, &g->n_fwd_big}};
+  };
+  PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
+                      {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
+                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
   for (auto &l : lists) {
     *l.count = static_cast<uint32_t>(l.v->size());
     if (l.v->empty()) continue;
@@ -224,15 +256,31 @@ int acc_graph_info(const acc_graph *g, uint32_t *n, uint64_t *m, uint64_t *devic
 
 int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   if (!g) return acc_set_error(ACC_EINVAL, "null graph");
-  if (path == ACC_PATH_AUTO) path = ACC_PATH_FORWARD;
-  if (path != ACC_PATH_FORWARD) return acc_set_error(ACC_EINVAL, "unknown counting path " + std::to_string(path));
+  if (path == ACC_PATH_AUTO) path = g->auto_path;
+  if (path != ACC_PATH_FORWARD && path != ACC_PATH_BITMAP && path != ACC_PATH_MERGE)
+    return acc_set_error(ACC_EINVAL, "unknown counting path " + std::to_string(path));
   DeviceGuard guard(g->ctx->device);
   const bool timing = (flags & ACC_CN_TIMING) != 0;
+  const int plus = (flags & ACC_CN_RAW) ? 0 : 2;
   for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
   g->events.clear();
   g->event_names.clear();
   g->timing_valid = false;
+  if (path == ACC_PATH_BITMAP) ACC_TRY(ensure_bitmap_pool(g));
   ACC_TRY(record(g, timing, "begin"));
+  if (path == ACC_PATH_BITMAP || path == ACC_PATH_MERGE) {
+    if (path == ACC_PATH_BITMAP) {
+      ACC_TRY(accpaths::launch_bitmap(g, plus));
+      ACC_TRY(record(g, timing, "k_bmp"));
+    } else {
+      ACC_TRY(accpaths::launch_merge(g, plus));
+      ACC_TRY(record(g, timing, "k_merge"));
+    }
+    g->have_result = true;
+    g->cnt_plus2 = plus == 2;
+    g->timing_valid = timing;
+    return ACC_OK;
+  }
   ACC_TRY(accfwd::launch_init(g));
   ACC_TRY(record(g, timing, "k_init"));
   ACC_TRY(accfwd::launch_count(g));

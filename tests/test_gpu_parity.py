@@ -20,7 +20,7 @@ from acctricnt_amd import graph as G
 pytestmark = pytest.mark.gpu
 
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
-PATHS = ["forward"]
+PATHS = ["forward", "bitmap", "merge"]
 
 
 def sha(a):
@@ -162,6 +162,21 @@ def test_malformed_upload_rejected(ctx, case):
     with pytest.raises(_ffi.AccError) as e:
         ctx.upload(g)
     assert e.value.code == -2
+
+
+def test_paths_agree_rmat20(ctx):
+    """R-MAT scale 20: the forward, bitmap and merge paths give identical counts,
+    with T equal to the oracle's triangle count."""
+    g = G.gen_rmat(20, 16, seed=3)
+    print(f"[paths rmat20] n={g.n} m={g.m}", flush=True)
+    res = {}
+    with ctx.upload(g) as rg:
+        for p in PATHS:
+            res[p] = rg.count_all_edges(p)
+    t = oracle.triangle_count(g)
+    for p in PATHS:
+        assert np.array_equal(res[p][0], res["forward"][0]), p
+        assert res[p][1] == t, p
 
 
 def test_full_size_rmat22_properties(ctx):
