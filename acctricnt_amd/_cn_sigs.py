@@ -28,6 +28,11 @@ SIGNATURES = {
     "acc_triangle_total": (C.c_int, [vp, C.c_int, u64p]),
     "acc_graph_set_param": (C.c_int, [vp, C.c_int, C.c_int64]),
     "acc_graph_get_param": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64)]),
+    "acc_graph_set_partition": (C.c_int, [vp, C.c_uint32, u32p]),
+    "acc_cn_graph_totals": (C.c_int, [vp, u64p]),
+    "acc_cn_capture": (C.c_int, [vp, C.c_int, C.c_uint32]),
+    "acc_cn_replay": (C.c_int, [vp]),
+    "acc_tune": (C.c_int, [vp, C.c_uint32, C.c_char_p, C.c_int]),
     "acc_graph_set_vertex_ids": (C.c_int, [vp, i32p]),
     "acc_scan_ws_create": (C.c_int, [vp, C.POINTER(vp)]),
     "acc_scan_ws_destroy": (None, [vp]),

@@ -26,8 +26,9 @@ PATH_BITMAP = 2
 PATH_MERGE = 4
 PATHS = {"auto": PATH_AUTO, "forward": PATH_FORWARD, "bitmap": PATH_BITMAP, "merge": PATH_MERGE}
 
-PARAM_FWD_UNROLL = 1
-PARAMS = {"fwd_unroll": PARAM_FWD_UNROLL}
+PARAMS = {"fwd_unroll": 1, "wave_slots": 2, "wave_low": 3, "chunk": 4, "auto_path": 5, "persist": 6}
+TUNE_QUICK = 0x1
+TUNE_FORWARD_ONLY = 0x2
 
 CN_RAW = 0x1
 CN_TIMING = 0x2
@@ -108,6 +109,7 @@ class ResidentGraph:
                                             _ffi.ptr(adj, C.c_int32), flags, C.byref(h)))
         self._h = h
         self._workspaces = weakref.WeakSet()
+        self.num_parts = 0
 
     @property
     def handle(self):
@@ -151,6 +153,15 @@ class ResidentGraph:
         keys = names.value.decode().split(";") if k else []
         return {key: float(ms[i]) for i, key in enumerate(keys)}
 
+    def tune(self, quick: bool = False, forward_only: bool = False) -> dict:
+        """Run the autotuner; returns its report (chosen configuration and every trial)."""
+        import json
+
+        buf = C.create_string_buffer(1 << 16)
+        flags = (TUNE_QUICK if quick else 0) | (TUNE_FORWARD_ONLY if forward_only else 0)
+        check_cn(self._lib.acc_tune(self._h, flags, buf, 1 << 16))
+        return json.loads(buf.value.decode())
+
     def set_param(self, name, value: int) -> None:
         key = PARAMS[name] if isinstance(name, str) else int(name)
         check_cn(self._lib.acc_graph_set_param(self._h, key, int(value)))
@@ -160,6 +171,30 @@ class ResidentGraph:
         v = C.c_int64()
         check_cn(self._lib.acc_graph_get_param(self._h, key, C.byref(v)))
         return int(v.value)
+
+    def set_partition(self, vertex_off) -> None:
+        """Declare the graph a packed multi-graph CSR (graph k owns vertices
+        [vertex_off[k], vertex_off[k+1])); counting passes then also reduce
+        per-graph triangle totals."""
+        vo = np.ascontiguousarray(vertex_off, dtype=np.uint32)
+        k = max(int(vo.shape[0]) - 1, 0)
+        self.num_parts = 0  # a rejected partition leaves none set
+        check_cn(self._lib.acc_graph_set_partition(self._h, k, _ffi.ptr(vo, C.c_uint32)))
+        self.num_parts = k
+
+    def graph_totals(self) -> np.ndarray:
+        """Per-graph triangle totals of the last counting pass (uint64[num_graphs])."""
+        t = np.empty(self.num_parts, dtype=np.uint64)
+        check_cn(self._lib.acc_cn_graph_totals(self._h, _ffi.ptr(t, C.c_uint64)))
+        return t
+
+    def capture(self, path="auto", raw: bool = False) -> None:
+        """Capture one counting pass into a HIP graph (replayed by :meth:`replay`)."""
+        check_cn(self._lib.acc_cn_capture(self._h, _path(path), CN_RAW if raw else 0))
+
+    def replay(self) -> None:
+        """Enqueue the captured counting pass (one graph launch, asynchronous)."""
+        check_cn(self._lib.acc_cn_replay(self._h))
 
     def device_counts_ptr(self) -> int:
         return self._lib.acc_cn_device_counts(self._h) or 0

@@ -32,6 +32,20 @@ struct acc_ctx {
   size_t lds_per_block = 0;
 };
 
+namespace accfwd {
+constexpr int kWG = 512;                              // threads per workgroup (8 waves)
+constexpr int kWaves = kWG / 64;
+constexpr int kTableBits = 12;                        // workgroup LDS hash table of 4096 keys
+constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
+constexpr uint32_t kChunk = 512;                      // upper slots per workgroup unit
+constexpr uint32_t kBitmapIds = 32u << kTableBits;    // b - min N(b) <= 2^17: N<(b) as an LDS bitmap of [min N(b), b)
+constexpr uint32_t kBitmapMaxLow = 65535;             // word prefixes are 16-bit
+constexpr int kWaveTableBits = 9;                     // wave-private table of 512 keys
+constexpr uint32_t kWaveCap = 1u << (kWaveTableBits - 1);  // |N<(b)| up to 256 in a wave unit
+constexpr uint32_t kWaveSlots = 64;                   // upper slots of a wave unit
+constexpr int kBigWG = 256;
+}  // namespace accfwd
+
 // One forward-path work unit: source b, its upper slots [j0, j1), nlow = |N<(b)|.
 struct FwdUnit {
   int32_t b;
@@ -77,6 +91,25 @@ struct acc_graph {
   // tuning parameters (acc_graph_set_param)
   int fwd_unroll = 4;
   int auto_path = ACC_PATH_FORWARD;  // what ACC_PATH_AUTO runs
+  uint32_t wave_slots = accfwd::kWaveSlots;  // wave-unit bin: at most this many upper slots
+  uint32_t wave_low = accfwd::kWaveCap;      // ... and |N<(b)| at most this
+  uint32_t chunk = accfwd::kChunk;           // upper slots per workgroup unit
+  uint32_t persist = 0;                      // counting grid: 0 one workgroup per task, else per-CU workgroups
+  // host copies kept for plan rebuilds: offsets, first upper slot and smallest
+  // neighbor (the vertex itself when it has no lower neighbor) of every vertex
+  std::vector<uint32_t> h_off;
+  std::vector<uint32_t> h_hs;
+  std::vector<int32_t> h_lo;
+  // multi-graph partition: graph k owns vertices [gvoff[k], gvoff[k+1])
+  uint32_t num_parts = 0;
+  uint32_t *d_gvoff = nullptr;
+  uint32_t *d_gsoff = nullptr;           // slot offsets: graph k owns slots [gsoff[k], gsoff[k+1])
+  unsigned long long *d_gtot = nullptr;  // per-graph sum of raw counts (= 6 T_k)
+  bool totals_valid = false;             // d_gtot holds the last pass's totals
+  // captured counting pass
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_plus2 = true;  // count format the captured pass writes
   // last results / timing
   bool have_result = false;
   std::vector<hipEvent_t> events;
@@ -95,21 +128,11 @@ int acc_graph_ensure_src(acc_graph *g);
 
 // Kernel launchers (k_forward.hip).
 namespace accfwd {
-constexpr int kWG = 512;                              // threads per workgroup (8 waves)
-constexpr int kWaves = kWG / 64;
-constexpr int kTableBits = 12;                        // workgroup LDS hash table of 4096 keys
-constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
-constexpr uint32_t kChunk = 512;                      // upper slots per workgroup unit
-constexpr uint32_t kBitmapIds = 32u << kTableBits;    // sources b <= 2^17: N<(b) as an LDS bitmap of [0, b)
-constexpr uint32_t kBitmapMaxLow = 65535;             // word prefixes are 16-bit
-constexpr int kWaveTableBits = 9;                     // wave-private table of 512 keys
-constexpr uint32_t kWaveCap = 1u << (kWaveTableBits - 1);  // |N<(b)| up to 256 in a wave unit
-constexpr uint32_t kWaveSlots = 64;                   // upper slots of a wave unit
-constexpr int kBigWG = 256;
 int launch_init(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_count_big(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);
+int launch_graph_totals(acc_graph *g, int plus);
 int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
                     hipStream_t s);
 }  // namespace accfwd

@@ -38,54 +38,99 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
                   static_cast<void *>(g->d_bmp_pool)})
     if (p) (void)hipFree(p);
+  if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->d_gvoff) (void)hipFree(g->d_gvoff);
+  if (g->d_gsoff) (void)hipFree(g->d_gsoff);
+  if (g->d_gtot) (void)hipFree(g->d_gtot);
   for (hipEvent_t e : g->events) (void)hipEventDestroy(e);
   delete g;
 }
 
 // Forward plan, binned by degree skew: a source b with upper slots (neighbors
-// > b) becomes one wave unit when it has at most kWaveSlots upper slots and
-// |N<(b)| <= kWaveCap, otherwise workgroup units of at most kChunk upper
-// slots (N<(b) as an LDS bitmap for b <= kBitmapIds, else an LDS hash
-// table); sources whose N<(b) fits neither go to the global-memory variant.
-void build_forward_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &wave,
-                        std::vector<FwdUnit> &block, std::vector<FwdUnit> &big, uint32_t &max_low,
-                        uint32_t &max_deg) {
-  max_low = 0;
-  max_deg = 0;
+// > b) becomes one wave unit when it has at most wave_slots upper slots and
+// |N<(b)| <= wave_low, otherwise workgroup units of at most `chunk` upper
+// slots (N<(b) as an LDS bitmap when b - lo[b] <= kBitmapIds, else an LDS
+// hash table); sources whose N<(b) fits neither go to the global-memory
+// variant.  hs[b] is the first upper slot of b, lo[b] its smallest neighbor.
+void build_forward_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, const int32_t *lo, uint32_t wave_slots,
+                        uint32_t wave_low, uint32_t chunk, std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block,
+                        std::vector<FwdUnit> &big) {
   for (uint32_t b = 0; b < n; b++) {
-    const int32_t *lb = std::lower_bound(adj + off[b], adj + off[b + 1], static_cast<int32_t>(b));
-    const uint32_t hs = static_cast<uint32_t>(lb - adj);
-    const uint32_t nl = hs - off[b];
-    max_low = std::max(max_low, nl);
-    max_deg = std::max(max_deg, off[b + 1] - off[b]);
-    const uint32_t nup = off[b + 1] - hs;
+    const uint32_t nl = hs[b] - off[b];
+    const uint32_t nup = off[b + 1] - hs[b];
     if (nup == 0) continue;
-    if (nup <= accfwd::kWaveSlots && nl <= accfwd::kWaveCap) {
-      wave.push_back(FwdUnit{static_cast<int32_t>(b), hs, off[b + 1], nl});
+    if (nup <= wave_slots && nl <= wave_low) {
+      wave.push_back(FwdUnit{static_cast<int32_t>(b), hs[b], off[b + 1], nl});
       continue;
     }
-    for (uint32_t j0 = hs; j0 < off[b + 1]; j0 += accfwd::kChunk) {
-      FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + accfwd::kChunk, off[b + 1]), nl};
-      const bool lds = nl <= accfwd::kLdsCap || (b <= accfwd::kBitmapIds && nl <= accfwd::kBitmapMaxLow);
+    for (uint32_t j0 = hs[b]; j0 < off[b + 1]; j0 += chunk) {
+      FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + chunk, off[b + 1]), nl};
+      const uint32_t span = b - static_cast<uint32_t>(lo[b]);
+      const bool lds = nl <= accfwd::kLdsCap || (span <= accfwd::kBitmapIds && nl <= accfwd::kBitmapMaxLow);
       (lds ? block : big).push_back(u);
     }
   }
 }
 
-// Per-edge plan: chunks of at most kChunk upper slots of every source (merge
+// Per-edge plan: chunks of at most `chunk` upper slots of every source (merge
 // path); the bitmap path splits them by d_u into LDS-set and hub units.
-void build_edge_plan(uint32_t n, const uint32_t *off, const int32_t *adj, std::vector<FwdUnit> &all,
+void build_edge_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, uint32_t chunk, std::vector<FwdUnit> &all,
                      std::vector<FwdUnit> &lds, std::vector<FwdUnit> &hub) {
   for (uint32_t u = 0; u < n; u++) {
-    const int32_t *ub = std::upper_bound(adj + off[u], adj + off[u + 1], static_cast<int32_t>(u));
-    const uint32_t hs = static_cast<uint32_t>(ub - adj);
     const uint32_t du = off[u + 1] - off[u];
-    for (uint32_t j0 = hs; j0 < off[u + 1]; j0 += accfwd::kChunk) {
-      FwdUnit un{static_cast<int32_t>(u), j0, std::min(j0 + accfwd::kChunk, off[u + 1]), du};
+    for (uint32_t j0 = hs[u]; j0 < off[u + 1]; j0 += chunk) {
+      FwdUnit un{static_cast<int32_t>(u), j0, std::min(j0 + chunk, off[u + 1]), du};
       all.push_back(un);
       (du <= accfwd::kLdsCap ? lds : hub).push_back(un);
     }
   }
+}
+
+template <typename T>
+void dev_free(acc_graph *g, T **p, uint64_t count) {
+  if (*p) {
+    (void)hipFree(*p);
+    g->device_bytes -= sizeof(T) * (count ? count : 1);
+  }
+  *p = nullptr;
+}
+
+// Drops the captured counting pass: it holds the addresses and launch shapes
+// of the plans and partition arrays, so any change to those invalidates it.
+void drop_capture(acc_graph *g) {
+  if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->graph_exec = nullptr;
+  g->graph = nullptr;
+}
+
+// (Re)builds every work plan from the host split points and the current bin
+// parameters, and uploads the unit lists.
+int upload_plans(acc_graph *g) {
+  drop_capture(g);
+  std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
+  build_forward_plan(g->n, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots, g->wave_low, g->chunk, wave, block, big);
+  build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
+  struct PlanList {
+    std::vector<FwdUnit> *v;
+    FwdUnit **d;
+    uint32_t *count;
+  };
+  PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
+                      {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
+                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  for (auto &l : lists) {
+    dev_free(g, l.d, *l.count);
+    *l.count = static_cast<uint32_t>(l.v->size());
+    if (l.v->empty()) continue;
+    ACC_TRY(dev_alloc(g, l.d, l.v->size()));
+    ACC_HIP_TRY(hipMemcpyAsync(*l.d, l.v->data(), sizeof(FwdUnit) * l.v->size(), hipMemcpyHostToDevice,
+                               g->ctx->stream));
+  }
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  return ACC_OK;
 }
 
 int ensure_bitmap_pool(acc_graph *g) {
@@ -220,26 +265,18 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
     if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
-  std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
-  build_forward_plan(n, node_off, edge_dst, wave, block, big, g->max_low, g->max_deg);
-  build_edge_plan(n, node_off, edge_dst, eall, elds, ehub);
-  struct PlanList {
-    std::vector<FwdUnit> *v;
-    FwdUnit **d;
-    uint32_t *count;
-  };
-  PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
-                      {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
-                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
-  for (auto &l : lists) {
-    *l.count = static_cast<uint32_t>(l.v->size());
-    if (l.v->empty()) continue;
-    if ((rc = dev_alloc(g, l.d, l.v->size()))) return bail(rc);
-    e = hipMemcpyAsync(*l.d, l.v->data(), sizeof(FwdUnit) * l.v->size(), hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
+  g->h_off.assign(node_off, node_off + static_cast<size_t>(n) + 1);
+  g->h_hs.resize(n);
+  g->h_lo.resize(n);
+  for (uint32_t v = 0; v < n; v++) {
+    const int32_t *ub = std::upper_bound(edge_dst + node_off[v], edge_dst + node_off[v + 1], static_cast<int32_t>(v));
+    g->h_hs[v] = static_cast<uint32_t>(ub - edge_dst);
+    g->h_lo[v] = node_off[v + 1] > node_off[v] ? std::min(edge_dst[node_off[v]], static_cast<int32_t>(v))
+                                                 : static_cast<int32_t>(v);
+    g->max_low = std::max(g->max_low, g->h_hs[v] - node_off[v]);
+    g->max_deg = std::max(g->max_deg, node_off[v + 1] - node_off[v]);
   }
-  e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, hipGetErrorString(e)));
+  if ((rc = upload_plans(g))) return bail(rc);
   *out = g;
   return ACC_OK;
 }
@@ -276,6 +313,11 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
       ACC_TRY(accpaths::launch_merge(g, plus));
       ACC_TRY(record(g, timing, "k_merge"));
     }
+    if (g->num_parts) {
+      ACC_TRY(accfwd::launch_graph_totals(g, plus));
+      ACC_TRY(record(g, timing, "k_graph_totals"));
+    }
+    g->totals_valid = g->num_parts != 0;
     g->have_result = true;
     g->cnt_plus2 = plus == 2;
     g->timing_valid = timing;
@@ -291,6 +333,11 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   }
   ACC_TRY(accfwd::launch_combine(g, (flags & ACC_CN_RAW) ? 0 : 2));
   ACC_TRY(record(g, timing, "k_combine"));
+  if (g->num_parts) {
+    ACC_TRY(accfwd::launch_graph_totals(g, (flags & ACC_CN_RAW) ? 0 : 2));
+    ACC_TRY(record(g, timing, "k_graph_totals"));
+  }
+  g->totals_valid = g->num_parts != 0;
   g->have_result = true;
   g->cnt_plus2 = (flags & ACC_CN_RAW) == 0;
   g->timing_valid = timing;
@@ -340,10 +387,36 @@ int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap)
 
 int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
   if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  DeviceGuard guard(g->ctx->device);
+  drop_capture(g);
   switch (param) {
     case ACC_PARAM_FWD_UNROLL:
       if (value != 1 && value != 2 && value != 4) return acc_set_error(ACC_EINVAL, "fwd_unroll must be 1, 2 or 4");
       g->fwd_unroll = static_cast<int>(value);
+      return ACC_OK;
+    case ACC_PARAM_WAVE_SLOTS:
+      if (value < 0 || value > static_cast<int64_t>(accfwd::kWaveSlots))
+        return acc_set_error(ACC_EINVAL, "wave_slots must lie in [0, 64]");
+      g->wave_slots = static_cast<uint32_t>(value);
+      return upload_plans(g);
+    case ACC_PARAM_WAVE_LOW:
+      if (value < 0 || value > static_cast<int64_t>(accfwd::kWaveCap))
+        return acc_set_error(ACC_EINVAL, "wave_low must lie in [0, 256]");
+      g->wave_low = static_cast<uint32_t>(value);
+      return upload_plans(g);
+    case ACC_PARAM_CHUNK:
+      if (value < 64 || value > static_cast<int64_t>(accfwd::kChunk))
+        return acc_set_error(ACC_EINVAL, "chunk must lie in [64, 512]");
+      g->chunk = static_cast<uint32_t>(value);
+      return upload_plans(g);
+    case ACC_PARAM_PERSIST:
+      if (value < 0 || value > 64) return acc_set_error(ACC_EINVAL, "persist must lie in [0, 64]");
+      g->persist = static_cast<uint32_t>(value);
+      return ACC_OK;
+    case ACC_PARAM_AUTO_PATH:
+      if (value != ACC_PATH_FORWARD && value != ACC_PATH_BITMAP && value != ACC_PATH_MERGE)
+        return acc_set_error(ACC_EINVAL, "auto_path must be a counting path");
+      g->auto_path = static_cast<int>(value);
       return ACC_OK;
     default:
       return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
@@ -353,12 +426,194 @@ int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
 int acc_graph_get_param(const acc_graph *g, int param, int64_t *value) {
   if (!g || !value) return acc_set_error(ACC_EINVAL, "null argument");
   switch (param) {
-    case ACC_PARAM_FWD_UNROLL:
-      *value = g->fwd_unroll;
-      return ACC_OK;
-    default:
-      return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
+    case ACC_PARAM_FWD_UNROLL: *value = g->fwd_unroll; return ACC_OK;
+    case ACC_PARAM_WAVE_SLOTS: *value = g->wave_slots; return ACC_OK;
+    case ACC_PARAM_WAVE_LOW: *value = g->wave_low; return ACC_OK;
+    case ACC_PARAM_CHUNK: *value = g->chunk; return ACC_OK;
+    case ACC_PARAM_AUTO_PATH: *value = g->auto_path; return ACC_OK;
+    case ACC_PARAM_PERSIST: *value = g->persist; return ACC_OK;
+    default: return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
   }
+}
+
+namespace {
+// Device time (ms) of one counting pass with the current parameters: the
+// median of `reps` passes after one warm-up pass.
+int time_pass(acc_graph *g, int path, int reps, float *ms) {
+  std::vector<float> t;
+  for (int r = 0; r <= reps; r++) {
+    ACC_TRY(acc_cn_compute(g, path, ACC_CN_TIMING));
+    float k[32];
+    const int nk = acc_cn_timings(g, k, 32, nullptr, 0);
+    if (nk < 0) return nk;
+    float tot = 0;
+    for (int i = 0; i < nk && i < 32; i++) tot += k[i];
+    if (r > 0) t.push_back(tot);
+  }
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return ACC_OK;
+}
+}  // namespace
+
+int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  DeviceGuard guard(g->ctx->device);
+  const bool quick = (flags & ACC_TUNE_QUICK) != 0;
+  const int reps = quick ? 1 : 2;
+  // Coordinate descent over the forward path's parameters, one at a time in
+  // this order, each trial timed on the device; then the per-edge paths at
+  // the chosen chunk size.
+  struct Axis {
+    int param;
+    const char *name;
+    std::vector<int64_t> values;
+  };
+  const std::vector<Axis> axes = {
+      {ACC_PARAM_FWD_UNROLL, "fwd_unroll", quick ? std::vector<int64_t>{2, 4} : std::vector<int64_t>{1, 2, 4}},
+      {ACC_PARAM_WAVE_SLOTS, "wave_slots", quick ? std::vector<int64_t>{16, 64} : std::vector<int64_t>{0, 16, 32, 64}},
+      {ACC_PARAM_WAVE_LOW, "wave_low", quick ? std::vector<int64_t>{256} : std::vector<int64_t>{64, 128, 256}},
+      {ACC_PARAM_CHUNK, "chunk", quick ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{128, 256, 512}},
+      {ACC_PARAM_PERSIST, "persist", quick ? std::vector<int64_t>{0, 4} : std::vector<int64_t>{0, 2, 4, 8}},
+  };
+  std::string trials;
+  auto add_trial = [&](const std::string &path, const char *param, int64_t value, float ms) {
+    trials += (trials.empty() ? "" : ", ") + std::string("[\"") + path + "\", \"" + param + "\", " +
+              std::to_string(value) + ", " + std::to_string(ms) + "]";
+  };
+  float best_ms = 0;
+  ACC_TRY(time_pass(g, ACC_PATH_FORWARD, reps, &best_ms));
+  add_trial("forward", "initial", 0, best_ms);
+  for (const Axis &ax : axes) {
+    int64_t keep = 0;
+    ACC_TRY(acc_graph_get_param(g, ax.param, &keep));
+    for (int64_t v : ax.values) {
+      if (v == keep) continue;
+      ACC_TRY(acc_graph_set_param(g, ax.param, v));
+      float ms = 0;
+      ACC_TRY(time_pass(g, ACC_PATH_FORWARD, reps, &ms));
+      add_trial("forward", ax.name, v, ms);
+      if (ms < best_ms) {
+        best_ms = ms;
+        keep = v;
+      }
+    }
+    ACC_TRY(acc_graph_set_param(g, ax.param, keep));
+  }
+  int best_path = ACC_PATH_FORWARD;
+  if (!(flags & ACC_TUNE_FORWARD_ONLY)) {
+    for (int path : {ACC_PATH_BITMAP, ACC_PATH_MERGE}) {
+      float ms = 0;
+      ACC_TRY(time_pass(g, path, 1, &ms));
+      add_trial(path == ACC_PATH_BITMAP ? "bitmap" : "merge", "path", path, ms);
+      if (ms < best_ms) {
+        best_ms = ms;
+        best_path = path;
+      }
+    }
+  }
+  ACC_TRY(acc_graph_set_param(g, ACC_PARAM_AUTO_PATH, best_path));
+  if (report && report_cap > 0) {
+    std::string r = std::string("{\"path\": \"") +
+                    (best_path == ACC_PATH_FORWARD ? "forward" : best_path == ACC_PATH_BITMAP ? "bitmap" : "merge") +
+                    "\"";
+    for (const Axis &ax : axes) {
+      int64_t v = 0;
+      ACC_TRY(acc_graph_get_param(g, ax.param, &v));
+      r += ", \"" + std::string(ax.name) + "\": " + std::to_string(v);
+    }
+    r += ", \"ms\": " + std::to_string(best_ms) + ", \"trials\": [" + trials + "]}";
+    std::strncpy(report, r.c_str(), static_cast<size_t>(report_cap) - 1);
+    report[report_cap - 1] = '\0';
+  }
+  return ACC_OK;
+}
+
+int acc_graph_set_partition(acc_graph *g, uint32_t num_graphs, const uint32_t *vertex_off) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  DeviceGuard guard(g->ctx->device);
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  drop_capture(g);
+  dev_free(g, &g->d_gvoff, g->num_parts + 1ULL);
+  dev_free(g, &g->d_gsoff, g->num_parts + 1ULL);
+  dev_free(g, &g->d_gtot, g->num_parts);
+  g->num_parts = 0;
+  g->totals_valid = false;
+  if (num_graphs == 0) return ACC_OK;
+  if (!vertex_off) return acc_set_error(ACC_EINVAL, "null vertex offsets");
+  if (vertex_off[0] != 0 || vertex_off[num_graphs] != g->n)
+    return acc_set_error(ACC_EINVAL, "vertex offsets must start at 0 and end at n");
+  for (uint32_t k = 0; k < num_graphs; k++)
+    if (vertex_off[k + 1] < vertex_off[k]) return acc_set_error(ACC_EINVAL, "vertex offsets must not decrease");
+  std::vector<uint32_t> gsoff(num_graphs + 1ULL);
+  for (uint32_t k = 0; k <= num_graphs; k++) gsoff[k] = g->h_off[vertex_off[k]];
+  // No edge may join two graphs.  In a symmetric CSR an edge (u, v) with u in
+  // graph k and v in a later graph puts u below v's range, so checking every
+  // vertex's smallest neighbor against its own range's start suffices.
+  for (uint32_t k = 0; k < num_graphs; k++)
+    for (uint32_t v = vertex_off[k]; v < vertex_off[k + 1]; v++)
+      if (static_cast<uint32_t>(g->h_lo[v]) < vertex_off[k])
+        return acc_set_error(ACC_EINVAL, "an edge joins two graphs of the partition");
+  ACC_TRY(dev_alloc(g, &g->d_gvoff, static_cast<uint64_t>(num_graphs) + 1));
+  ACC_TRY(dev_alloc(g, &g->d_gsoff, static_cast<uint64_t>(num_graphs) + 1));
+  ACC_TRY(dev_alloc(g, &g->d_gtot, num_graphs));
+  ACC_HIP_TRY(hipMemcpyAsync(g->d_gvoff, vertex_off, sizeof(uint32_t) * (num_graphs + 1ULL), hipMemcpyHostToDevice,
+                             g->ctx->stream));
+  ACC_HIP_TRY(hipMemcpyAsync(g->d_gsoff, gsoff.data(), sizeof(uint32_t) * (num_graphs + 1ULL), hipMemcpyHostToDevice,
+                             g->ctx->stream));
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  g->num_parts = num_graphs;
+  return ACC_OK;
+}
+
+int acc_cn_graph_totals(acc_graph *g, uint64_t *totals) {
+  if (!g || !totals) return acc_set_error(ACC_EINVAL, "null argument");
+  if (!g->num_parts) return acc_set_error(ACC_EINVAL, "no partition set");
+  if (!g->totals_valid) return acc_set_error(ACC_EINVAL, "no counting pass since the partition was set");
+  DeviceGuard guard(g->ctx->device);
+  std::vector<unsigned long long> t(g->num_parts);
+  ACC_HIP_TRY(hipMemcpyAsync(t.data(), g->d_gtot, sizeof(unsigned long long) * g->num_parts, hipMemcpyDeviceToHost,
+                             g->ctx->stream));
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  for (uint32_t k = 0; k < g->num_parts; k++) totals[k] = t[k] / 6;
+  return ACC_OK;
+}
+
+int acc_cn_capture(acc_graph *g, int path, uint32_t flags) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (flags & ACC_CN_TIMING) return acc_set_error(ACC_EINVAL, "timing events cannot be captured");
+  DeviceGuard guard(g->ctx->device);
+  if (path == ACC_PATH_AUTO) path = g->auto_path;
+  if (path == ACC_PATH_BITMAP) ACC_TRY(ensure_bitmap_pool(g));  // allocates and clears outside the capture
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  drop_capture(g);
+  ACC_HIP_TRY(hipStreamBeginCapture(g->ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = acc_cn_compute(g, path, flags);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(g->ctx->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return acc_set_error(ACC_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  g->graph = graph;
+  ACC_HIP_TRY(hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0));
+  g->graph_plus2 = !(flags & ACC_CN_RAW);
+  g->have_result = false;
+  g->totals_valid = false;
+  return ACC_OK;
+}
+
+int acc_cn_replay(acc_graph *g) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (!g->graph_exec) return acc_set_error(ACC_EINVAL, "no captured counting pass");
+  DeviceGuard guard(g->ctx->device);
+  ACC_HIP_TRY(hipGraphLaunch(g->graph_exec, g->ctx->stream));
+  g->cnt_plus2 = g->graph_plus2;
+  g->timing_valid = false;
+  g->have_result = true;
+  g->totals_valid = g->num_parts != 0;
+  return ACC_OK;
 }
 
 int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles) {

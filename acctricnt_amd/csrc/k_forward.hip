@@ -107,17 +107,21 @@ struct HashSet {
   __device__ __forceinline__ int find(int32_t a) const { return lds_find(keys, kpos, bits, mask, a); }
 };
 
-// Membership of N<(b) ⊂ [0, b) as an exact LDS bitmap over [0, b) with the
-// position of a in N<(b) recovered from the index of the first element of
-// a's 32-bit word (pre) plus a popcount: one LDS read decides a probe.
+// Membership of N<(b) ⊂ [lo, b), lo = min N(b), as an exact LDS bitmap over
+// [lo, b) with the position of a in N<(b) recovered from the index of the
+// first element of a's 32-bit word (pre) plus a popcount: one LDS read
+// decides a probe.  Callers only ask for a < b.
 struct BitSet {
   const uint32_t *bm;
   const uint16_t *pre;
+  int32_t lo;
   __device__ __forceinline__ int find(int32_t a) const {
-    const uint32_t w = bm[a >> 5];
-    const uint32_t bit = 1u << (a & 31);
+    const int32_t r = a - lo;
+    if (r < 0) return -1;
+    const uint32_t w = bm[r >> 5];
+    const uint32_t bit = 1u << (r & 31);
     if (!(w & bit)) return -1;
-    return pre[a >> 5] + __popc(w & (bit - 1));
+    return pre[r >> 5] + __popc(w & (bit - 1));
   }
 };
 
@@ -201,7 +205,8 @@ __device__ __forceinline__ void fwd_block_loop(const int32_t *__restrict__ adj, 
 }
 
 // Workgroup unit: one chunk of at most kChunk upper slots of a source b.
-// N<(b) is an LDS bitmap when b <= kBitmapIds (|N<(b)| < 65536), otherwise an
+// N<(b) is an LDS bitmap when b - min N(b) <= kBitmapIds (|N<(b)| < 65536),
+// otherwise an
 // LDS hash table (|N<(b)| <= kLdsCap).  Both layouts share the ctr and
 // neighbor-metadata regions of smem.
 template <int U>
@@ -218,11 +223,13 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   const uint32_t nl = u.nlow;
   const uint32_t nup = u.j1 - u.j0;
   const int tid = threadIdx.x;
-  const bool bitmap = static_cast<uint32_t>(b) <= accfwd::kBitmapIds;
+  const int32_t lo = nl ? adj[sb] : b;
+  const uint32_t span = static_cast<uint32_t>(b - lo);
+  const bool bitmap = span <= accfwd::kBitmapIds;
   const uint32_t nctr = nl < kLdsCap ? nl : kLdsCap;
   const int bits = bitmap ? 0 : table_bits_for(nl);
   const uint32_t mask = bitmap ? 0 : (1u << bits) - 1;
-  const uint32_t nclear = bitmap ? (static_cast<uint32_t>(b) + 31) / 32 : mask + 1;
+  const uint32_t nclear = bitmap ? (span + 31) / 32 : mask + 1;
   const int32_t fill = bitmap ? 0 : -1;
   for (uint32_t i = tid; i < nclear; i += kWG) keys[i] = fill;
   for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
@@ -235,16 +242,16 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   if (bitmap) {
     uint32_t *bm = reinterpret_cast<uint32_t *>(keys);
     for (uint32_t i = tid; i < nl; i += kWG) {
-      const int32_t a = adj[sb + i];
-      atomicOr(&bm[a >> 5], 1u << (a & 31));
-      if (i == 0 || (adj[sb + i - 1] >> 5) != (a >> 5)) kpos[a >> 5] = static_cast<uint16_t>(i);
+      const int32_t r = adj[sb + i] - lo;
+      atomicOr(&bm[r >> 5], 1u << (r & 31));
+      if (i == 0 || ((adj[sb + i - 1] - lo) >> 5) != (r >> 5)) kpos[r >> 5] = static_cast<uint16_t>(i);
     }
   } else {
     lds_insert(keys, kpos, bits, mask, adj, sb, nl, tid, kWG);
   }
   __syncthreads();
   if (bitmap) {
-    const BitSet set{reinterpret_cast<const uint32_t *>(keys), kpos};
+    const BitSet set{reinterpret_cast<const uint32_t *>(keys), kpos, lo};
     fwd_block_loop<U>(adj, u, cnt, rev, set, ctr, s_cb, s_ce, sb, nup);
   } else {
     const HashSet set{keys, kpos, bits, mask};
@@ -258,8 +265,7 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
 }
 
 // Wave unit: a source with at most kWaveSlots upper slots and
-// |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.  No
-// workgroup barrier may follow a call: waves without a unit return early.
+// |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.
 template <int U>
 __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                               const FwdUnit u, int32_t *cnt, uint32_t *__restrict__ rev,
@@ -303,8 +309,11 @@ __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, 
   }
 }
 
-// The forward counting launch: workgroups [0, n_block) take one workgroup
-// unit each, the following workgroups take kWaves wave units each.
+// The forward counting launch.  Tasks [0, n_block) are one workgroup unit
+// each, the following tasks kWaves wave units each; workgroup w runs tasks
+// w, w + gridDim.x, ... (one task per workgroup when the grid covers them, a
+// persistent grid otherwise).  The task index is workgroup-uniform, so the
+// barrier that separates consecutive tasks' use of LDS is reached by all.
 template <int U>
 __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
                                                    const int32_t *__restrict__ adj,
@@ -312,14 +321,17 @@ __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ 
                                                    const FwdUnit *__restrict__ wave_units, uint32_t n_wave,
                                                    int32_t *cnt, uint32_t *__restrict__ rev) {
   __shared__ __align__(16) unsigned char smem[kCountLdsBytes];
-  if (blockIdx.x < n_block) {
-    fwd_block_unit<U>(off, adj, block_units[blockIdx.x], cnt, rev, smem);
-    return;
-  }
+  const uint32_t tasks = n_block + (n_wave + kWaves - 1) / kWaves;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t ui = (blockIdx.x - n_block) * kWaves + wave;
-  if (ui >= n_wave) return;
-  fwd_wave_unit<U>(off, adj, wave_units[ui], cnt, rev, smem, wave);
+  for (uint32_t t = blockIdx.x; t < tasks; t += gridDim.x) {
+    if (t < n_block) {
+      fwd_block_unit<U>(off, adj, block_units[t], cnt, rev, smem);
+    } else {
+      const uint32_t ui = (t - n_block) * kWaves + wave;
+      if (ui < n_wave) fwd_wave_unit<U>(off, adj, wave_units[ui], cnt, rev, smem, wave);
+    }
+    __syncthreads();
+  }
 }
 
 // Sources whose N<(b) exceeds the workgroup table: membership by binary
@@ -399,6 +411,47 @@ __global__ __launch_bounds__(256) void k_combine(int32_t *__restrict__ cnt, cons
   }
 }
 
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long *__restrict__ p, uint32_t count) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// Per-graph sums of raw counts (cnt - plus) over a multi-graph CSR whose
+// graph k owns slots [gsoff[k], gsoff[k+1]).  Workgroup w sums the slots
+// [w * kTotSpan, (w + 1) * kTotSpan), split at graph boundaries, and adds
+// each piece to its graph's total.
+constexpr uint32_t kTotSpan = 1u << 16;
+__global__ __launch_bounds__(256) void k_graph_totals(const uint32_t *__restrict__ gsoff, uint32_t parts,
+                                                      uint32_t m, const int32_t *__restrict__ cnt, int plus,
+                                                      unsigned long long *__restrict__ gtot) {
+  __shared__ unsigned long long part[4];
+  uint32_t s0 = blockIdx.x * kTotSpan;
+  const uint32_t s1 = m - s0 < kTotSpan ? m : s0 + kTotSpan;
+  // k: the last graph with gsoff[k] <= s0
+  uint32_t lo = 0, hi = parts;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (gsoff[mid] <= s0) lo = mid; else hi = mid;
+  }
+  uint32_t k = __builtin_amdgcn_readfirstlane(lo);
+  while (s0 < s1) {
+    const uint32_t ge = __builtin_amdgcn_readfirstlane(gsoff[k + 1]);
+    const uint32_t e = ge < s1 ? ge : s1;
+    unsigned long long s = 0;
+    for (uint32_t i = s0 + threadIdx.x; i < e; i += 256) s += static_cast<unsigned long long>(cnt[i] - plus);
+    s = wave_sum_u64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+      if (t) atomicAdd(&gtot[k], t);
+    }
+    __syncthreads();
+    s0 = e;
+    k++;
+  }
+}
+
 // Flags: 1 id out of range, 2 self-loop, 4 unsorted, 8 asymmetric.
 __global__ __launch_bounds__(256) void k_validate(const uint32_t *__restrict__ off,
                                                   const int32_t *__restrict__ adj, uint32_t n, uint64_t m,
@@ -454,7 +507,9 @@ int launch_init(acc_graph *g) {
 
 int launch_count(acc_graph *g) {
   hipStream_t st = g->ctx->stream;
-  const unsigned grid = g->n_fwd_block + (g->n_fwd_wave + kWaves - 1) / kWaves;
+  const uint64_t tasks = g->n_fwd_block + (g->n_fwd_wave + kWaves - 1) / kWaves;
+  const uint64_t cap = static_cast<uint64_t>(g->persist) * static_cast<uint64_t>(g->ctx->num_cus);
+  const unsigned grid = static_cast<unsigned>(g->persist && cap < tasks ? cap : tasks);
   if (grid) {
     switch (g->fwd_unroll) {
       case 1:
@@ -488,6 +543,20 @@ int launch_combine(acc_graph *g, int plus) {
   hipLaunchKernelGGL(k_combine, dim3(grid_for(g->m, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_cnt,
                      g->d_rev, g->m, plus, g->d_sum);
   ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_graph_totals(acc_graph *g, int plus) {
+  if (!g->num_parts) return ACC_OK;
+  hipLaunchKernelGGL(k_zero_u64, dim3((g->num_parts + 255) / 256), dim3(256), 0, g->ctx->stream, g->d_gtot,
+                     g->num_parts);
+  ACC_HIP_TRY(hipGetLastError());
+  if (g->m) {
+    const uint32_t blocks = static_cast<uint32_t>((g->m + kTotSpan - 1) / kTotSpan);
+    hipLaunchKernelGGL(k_graph_totals, dim3(blocks), dim3(256), 0, g->ctx->stream, g->d_gsoff, g->num_parts,
+                       static_cast<uint32_t>(g->m), g->d_cnt, plus, g->d_gtot);
+    ACC_HIP_TRY(hipGetLastError());
+  }
   return ACC_OK;
 }
 

@@ -194,3 +194,28 @@ def batch_mix(num_graphs: int = 512, seed: int = 2024, n_lo_log2: float = 8.0,
         g.meta.update({"kind": kind, "n": n, "avg_deg": deg, "seed": s})
         graphs.append(g)
     return graphs
+
+
+def pack(graphs) -> tuple[CSRGraph, np.ndarray]:
+    """Packed multi-graph CSR: the disjoint union of ``graphs``, graph k owning
+    vertices ``[vertex_off[k], vertex_off[k+1])`` (its ids shifted by
+    ``vertex_off[k]``).  Returns the packed graph and the int32 graph offsets
+    (uint32[len(graphs)+1])."""
+    ns = np.array([g.n for g in graphs], dtype=np.int64)
+    ms = np.array([g.m for g in graphs], dtype=np.int64)
+    vertex_off = np.zeros(len(graphs) + 1, dtype=np.int64)
+    np.cumsum(ns, out=vertex_off[1:])
+    slot_off = np.zeros(len(graphs) + 1, dtype=np.int64)
+    np.cumsum(ms, out=slot_off[1:])
+    if vertex_off[-1] >= 2**31 or slot_off[-1] >= 2**32:
+        raise ValueError("packed graph exceeds int32 ids / uint32 offsets")
+    node_off = np.empty(int(vertex_off[-1]) + 1, dtype=np.uint32)
+    edge_dst = np.empty(int(slot_off[-1]), dtype=np.int32)
+    for k, g in enumerate(graphs):
+        v0, s0 = int(vertex_off[k]), int(slot_off[k])
+        node_off[v0:v0 + g.n] = g.node_off[:-1].astype(np.int64) + s0
+        edge_dst[s0:s0 + g.m] = g.edge_dst + v0
+    node_off[-1] = slot_off[-1]
+    packed = CSRGraph(node_off, edge_dst, name=f"pack{len(graphs)}",
+                      meta={"graphs": len(graphs), "slot_off": slot_off})
+    return packed, vertex_off.astype(np.uint32)

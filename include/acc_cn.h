@@ -92,8 +92,38 @@ int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap)
 
 /* Tuning parameters of a resident graph (the knobs the autotuner sets). */
 #define ACC_PARAM_FWD_UNROLL 1 /* 64-element chunks per scan round in the forward path: 1, 2 or 4 */
+#define ACC_PARAM_WAVE_SLOTS 2 /* forward bins: sources with at most this many upper slots (0..64) ... */
+#define ACC_PARAM_WAVE_LOW 3   /* ... and |N<(b)| at most this (0..256) run one source per wave */
+#define ACC_PARAM_CHUNK 4      /* upper slots per workgroup unit (64..512), all paths */
+#define ACC_PARAM_AUTO_PATH 5  /* the path ACC_PATH_AUTO runs */
+#define ACC_PARAM_PERSIST 6    /* forward counting grid: 0 one workgroup per task, k>0 at most k per CU (persistent) */
 int acc_graph_set_param(acc_graph *g, int param, int64_t value);
 int acc_graph_get_param(const acc_graph *g, int param, int64_t *value);
+
+/* Autotuner: coordinate descent over the forward path's parameters (unroll,
+ * wave-bin boundaries, chunk = upper slots per 8-wave workgroup, persistent
+ * grid), each trial timed on the device, then the bitmap and merge paths;
+ * keeps the fastest as the graph's ACC_PATH_AUTO configuration and writes a
+ * JSON report of every trial (ms per full pass, HIP events) into report (may
+ * be NULL). */
+#define ACC_TUNE_QUICK 0x1u        /* fewer candidates and repetitions */
+#define ACC_TUNE_FORWARD_ONLY 0x2u /* skip the bitmap and merge trials */
+int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap);
+
+/* Multi-graph batches ----------------------------------------------------- */
+/* A packed multi-graph CSR is an ordinary CSR whose graphs occupy disjoint
+ * vertex ranges: graph k owns vertices [vertex_off[k], vertex_off[k+1]) (int32
+ * graph offsets, vertex_off[num_graphs] = n) and no edge joins two ranges.
+ * With a partition set, every counting pass also reduces per-graph triangle
+ * totals, read with acc_cn_graph_totals (num_graphs entries). */
+int acc_graph_set_partition(acc_graph *g, uint32_t num_graphs, const uint32_t *vertex_off);
+int acc_cn_graph_totals(acc_graph *g, uint64_t *totals);
+
+/* Capture one counting pass (the same kernels acc_cn_compute enqueues) into a
+ * HIP graph, and replay it with a single launch on the context stream.  The
+ * captured pass contains no memset nodes (counters are cleared by kernels). */
+int acc_cn_capture(acc_graph *g, int path, uint32_t flags);
+int acc_cn_replay(acc_graph *g);
 
 /* Triangle count (SCAN_XP::TriCnt / TriCntSortedArray,
  * scan-query/scan_xp_triangle_cnt.cpp:778-957): runs the counting path and

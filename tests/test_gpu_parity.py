@@ -113,14 +113,35 @@ def test_edge_cases(ctx, path):
     g = G.from_edges(10, [2, 3, 2], [3, 7, 7])
     cnt, t = gpu_counts(ctx, g, path)
     assert (cnt == 3).all() and t == 1
-    # forward lists longer than the LDS hash table: K_2600 on ids below 2^17 (LDS bitmap
-    # with counters spilling to atomics) and K_2200 on ids above 2^17 (global-memory lookups)
+    # forward lists longer than the LDS counter array: K_2600 (LDS bitmap, counters
+    # spilling to atomics), also placed at ids above 2^17 (bitmap over [min N(b), b))
     for k, base in ((2600, 0), (2200, 140_000)):
         s, d = np.triu_indices(k, 1)
         g = G.from_edges(base + k, s + base, d + base)
         cnt, t = gpu_counts(ctx, g, path)
         assert (cnt == k).all()
         assert t == k * (k - 1) * (k - 2) // 6
+
+
+def wide_span_graph(base, n_far, n_near, n_up, seed):
+    """A source b = base whose lower neighbors span more than 2^17 ids (n_far
+    ids near 0 and n_near ids just below b) with n_up upper neighbors, each
+    joined to three random lower neighbors and to the next upper neighbor."""
+    rng = np.random.default_rng(seed)
+    low = np.concatenate([np.arange(n_far), np.arange(base - n_near, base)])
+    up = np.arange(base + 1, base + 1 + n_up)
+    src = [np.full(low.size + up.size, base), np.repeat(up, 3), up[:-1], low[:50]]
+    dst = [np.concatenate([low, up]), rng.choice(low, 3 * up.size), up[1:], low[-50:]]
+    return G.from_edges(base + n_up + 1, np.concatenate(src), np.concatenate(dst),
+                        name=f"wide{n_far}+{n_near}x{n_up}")
+
+
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("shape", [(100, 900, 700), (1000, 2000, 300), (10, 40, 40)])
+def test_wide_span_sources(ctx, path, shape):
+    # |N<(b)| = 1000 with a span > 2^17: LDS hash table; |N<(b)| = 3000: global-memory
+    # lookups; a small wide source: wave unit
+    check_against_oracle(ctx, wide_span_graph(150_000, *shape, seed=sum(shape)), path)
 
 
 @pytest.mark.parametrize("unroll", [1, 2, 4])
