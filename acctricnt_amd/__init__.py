@@ -8,11 +8,12 @@ RapidsAtHKUST/AccTriCnt.  Native code:
 * ``libacc_cn.so``    the HIP library behind the C ABI in include/acc_cn.h;
 * ``libacc_graph.so`` host CSR utilities (include/acc_graph.h).
 
-Python modules: :mod:`.graph` (CSR, generators, on-disk format) and
-:mod:`.cn` (contexts, resident graphs, counting).  Importing the package
+Python modules: :mod:`.graph` (CSR, generators, on-disk format),
+:mod:`.cn` (contexts, resident graphs, counting, batches, SCAN) and
+:mod:`.serve` (SCAN serving over resident graphs).  Importing the package
 loads no native library; the first call that needs one loads it and fails
 loudly if it is not built.
 """
-from . import cn, graph  # noqa: F401
+from . import cn, graph, serve  # noqa: F401
 
-__all__ = ["graph", "cn"]
+__all__ = ["graph", "cn", "serve"]

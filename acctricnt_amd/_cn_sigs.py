@@ -9,6 +9,21 @@ u64p = C.POINTER(C.c_uint64)
 f32p = C.POINTER(C.c_float)
 vp = C.c_void_p
 
+
+
+class ServerConfig(C.Structure):
+    """acc_server_config"""
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("streams", C.c_int32),
+                ("slots", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    """acc_server_stats"""
+    _fields_ = [(name, C.c_uint64) for name in (
+        "submitted", "completed", "failed", "cancelled", "batches", "batched_queries", "max_batch_seen",
+        "admissions", "evictions", "graph_bytes", "slot_bytes", "resident_graphs", "pending")]
+
+
 # name -> (restype, argtypes); kept in step with include/acc_cn.h
 SIGNATURES = {
     "acc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
@@ -41,6 +56,19 @@ SIGNATURES = {
     "acc_scan_synchronize": (C.c_int, [vp]),
     "acc_scan_results": (C.c_int, [vp, i32p, i32p, C.POINTER(C.c_uint8), i32p, C.POINTER(C.c_int64)]),
     "acc_scan_pairs": (C.c_int, [vp, i32p, C.c_uint64, u64p]),
+    "acc_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "acc_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+    "acc_server_destroy": (None, [vp]),
+    "acc_server_add_graph": (C.c_int, [vp, C.c_uint32, C.c_uint64, u32p, i32p, C.c_uint32, i32p]),
+    "acc_server_remove_graph": (C.c_int, [vp, C.c_int32]),
+    "acc_server_graph_info": (C.c_int, [vp, C.c_int32, u32p, u64p, C.POINTER(C.c_int), u64p]),
+    "acc_server_submit": (C.c_int, [vp, C.c_int32, C.c_int64, C.c_int64, C.c_int32, u64p]),
+    "acc_server_wait": (C.c_int, [vp, C.c_uint64, C.c_int32]),
+    "acc_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_int)]),
+    "acc_server_result": (C.c_int, [vp, C.c_uint64, C.POINTER(i32p), C.POINTER(i32p), u32p, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_double)]),
+    "acc_server_release": (C.c_int, [vp, C.c_uint64]),
+    "acc_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
 }
 
 

@@ -55,6 +55,7 @@ class Context:
         self._h = h
         self.device = device
         self._graphs = weakref.WeakSet()
+        self._servers = weakref.WeakSet()
 
     @property
     def handle(self):
@@ -72,6 +73,8 @@ class Context:
         return rg
 
     def close(self) -> None:
+        for sv in list(self._servers):
+            sv.close()
         for rg in list(self._graphs):
             rg.release()
         if self._h:

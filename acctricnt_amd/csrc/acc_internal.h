@@ -64,7 +64,7 @@ struct acc_graph {
   uint32_t *d_off = nullptr;
   int32_t *d_adj = nullptr;
   int32_t *d_cnt = nullptr;
-  uint32_t *d_rev = nullptr;            // scratch: reverse slot of every upper slot, ~0 elsewhere
+  uint32_t *d_rev = nullptr;            // reverse slot of every upper slot, ~0 elsewhere (built at upload)
   unsigned long long *d_sum = nullptr;  // sum over upper slots of the raw counts (= 3T)
   int32_t *d_flags = nullptr;           // validation flags
   int32_t *d_src = nullptr;             // source vertex of every slot (built for SCAN)
@@ -126,9 +126,66 @@ int launch_merge(acc_graph *g, int plus);
 // Builds the slot -> source array (acc_scan.hip).
 int acc_graph_ensure_src(acc_graph *g);
 
+// One SCAN query as the batched SCAN kernels see it (acc_scan.hip): the
+// resident graph's arrays, the query's buffers and its exact eps = p/q, mu.
+struct ScanJob {
+  const uint32_t *off;
+  const int32_t *adj;
+  const int32_t *src;
+  const int32_t *cnt;
+  const int32_t *old;  // caller vertex ids, or NULL
+  uint8_t *sim;
+  int32_t *core_cnt;
+  int32_t *label;
+  int32_t *parent;
+  int32_t *rmin;
+  int32_t *rmax;
+  int32_t *cl;
+  int32_t *cid;
+  unsigned long long *counters;  // core, cluster, hub, outlier, pairs
+  int2 *pairs;
+  unsigned long long pairs_cap;
+  unsigned long long p2, q2;
+  uint64_t m;
+  uint32_t n;
+  int32_t add;  // added to cnt to get 2 + |N(u) ∩ N(v)|
+  int32_t mu;
+  uint32_t vb0, vbn;  // workgroups of vertex-sized launches
+  uint32_t sb0, sbn;  // workgroups of slot-sized launches
+};
+
+// Per-query device buffers, grown on demand.
+struct ScanBuffers {
+  uint8_t *sim = nullptr;
+  int32_t *core_cnt = nullptr;
+  int32_t *label = nullptr;
+  int32_t *parent = nullptr;
+  int32_t *rmin = nullptr;
+  int32_t *rmax = nullptr;
+  int32_t *cl = nullptr;
+  int32_t *cid = nullptr;
+  unsigned long long *counters = nullptr;
+  uint32_t n_cap = 0;
+  uint64_t m_cap = 0;
+  uint64_t bytes = 0;
+};
+
+namespace accscan {
+void free_buffers(ScanBuffers *b);
+int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m);
+uint64_t buffer_bytes(uint32_t n, uint64_t m);
+int check_eps(int64_t eps_num, int64_t eps_den);
+ScanJob make_job(const acc_graph *g, const ScanBuffers &b, int64_t eps_num, int64_t eps_den, int32_t mu);
+void plan_jobs(ScanJob *jobs, int nj, int cus, uint32_t *vblocks, uint32_t *sblocks);
+// Enqueues the eight SCAN kernels over the nj jobs at d_jobs (plan_jobs'd copy).
+int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
+int launch_pairs(const ScanJob *d_jobs, int nj, uint32_t sblocks, hipStream_t s);
+}  // namespace accscan
+
 // Kernel launchers (k_forward.hip).
 namespace accfwd {
 int launch_init(acc_graph *g);
+int launch_build_rev(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_count_big(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);

@@ -265,6 +265,7 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
     if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
+  if ((rc = accfwd::launch_build_rev(g))) return bail(rc);
   g->h_off.assign(node_off, node_off + static_cast<size_t>(n) + 1);
   g->h_hs.resize(n);
   g->h_lo.resize(n);

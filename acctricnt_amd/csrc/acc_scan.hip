@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <new>
+#include <type_traits>
 
 #include "acc_internal.h"
 
@@ -63,25 +64,49 @@ __global__ __launch_bounds__(256) void k_fill_src(const uint32_t *__restrict__ o
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_init(int32_t *__restrict__ core_cnt, uint32_t n,
-                                                   unsigned long long *__restrict__ counters) {
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  for (uint32_t u = t; u < n; u += gridDim.x * 256u) core_cnt[u] = 0;
-  if (t < 8) counters[t] = 0;
+// Every SCAN kernel below runs a table of jobs (one job = one (graph, eps,
+// mu) query) in one launch: job j owns the workgroups [sb0, sb0 + sbn) of a
+// slot-sized launch and [vb0, vb0 + vbn) of a vertex-sized one, and those
+// workgroups stride over its m slots or n vertices.  The job of a workgroup
+// is found by a binary search of the table with workgroup-uniform values.
+__device__ __forceinline__ ScanJob job_of(const ScanJob *__restrict__ jobs, int nj, bool slots, uint32_t *local,
+                                          uint32_t *count) {
+  const uint32_t blk = blockIdx.x;
+  int lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((slots ? jobs[mid].sb0 : jobs[mid].vb0) <= blk) lo = mid; else hi = mid;
+  }
+  const ScanJob J = jobs[lo];
+  *local = blk - (slots ? J.sb0 : J.vb0);
+  *count = slots ? J.sbn : J.vbn;
+  return J;
+}
+
+__global__ __launch_bounds__(256) void k_sj_init(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const uint32_t t = lb * 256u + threadIdx.x;
+  for (uint32_t u = t; u < J.n; u += nb * 256u) J.core_cnt[u] = 0;
+  if (t < 8) J.counters[t] = 0;
 }
 
 // Similarity of every slot and per-vertex counts of similar neighbors.  Each
 // wave covers 64 consecutive slots; runs of equal source are reduced with a
 // ballot and added with one atomic per run.
-__global__ __launch_bounds__(kScanWG) void k_scan_sim(const uint32_t *__restrict__ off,
-                                                      const int32_t *__restrict__ adj,
-                                                      const int32_t *__restrict__ src,
-                                                      const int32_t *__restrict__ cnt, uint64_t m, int32_t add,
-                                                      unsigned long long p2, unsigned long long q2,
-                                                      uint8_t *__restrict__ sim, int32_t *__restrict__ core_cnt) {
+__global__ __launch_bounds__(kScanWG) void k_sj_sim(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const uint32_t *__restrict__ off = J.off;
+  const int32_t *__restrict__ adj = J.adj;
+  const int32_t *__restrict__ src = J.src;
+  const int32_t *__restrict__ cnt = J.cnt;
+  const uint64_t m = J.m;
+  const int32_t add = J.add;
+  const unsigned long long p2 = J.p2, q2 = J.q2;
   const int lane = threadIdx.x & 63;
-  const uint64_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * static_cast<uint64_t>(kScanWG) + threadIdx.x) >> 6);
-  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * (kScanWG / 64);
+  const uint64_t wave = __builtin_amdgcn_readfirstlane((static_cast<uint64_t>(lb) * kScanWG + threadIdx.x) >> 6);
+  const uint64_t nwaves = static_cast<uint64_t>(nb) * (kScanWG / 64);
   for (uint64_t base = wave * 64; base < m; base += nwaves * 64) {
     const uint64_t i = base + lane;
     const bool valid = i < m;
@@ -96,7 +121,7 @@ __global__ __launch_bounds__(kScanWG) void k_scan_sim(const uint32_t *__restrict
       const unsigned __int128 lhs = static_cast<unsigned __int128>(q2) * (c * c);
       const unsigned __int128 rhs = static_cast<unsigned __int128>(p2) * (du1 * dv1);
       s = lhs >= rhs;
-      sim[i] = s ? 1 : 0;
+      J.sim[i] = s ? 1 : 0;
     }
     const int32_t prev = __shfl_up(u, 1, 64);
     const bool head = valid && (lane == 0 || prev != u);
@@ -107,77 +132,73 @@ __global__ __launch_bounds__(kScanWG) void k_scan_sim(const uint32_t *__restrict
       const int end = above ? __builtin_ctzll(above) : 64;
       const unsigned long long run = (end == 64 ? ~0ull : ((1ull << end) - 1)) & ~((1ull << lane) - 1);
       const int k = __popcll(sims & run);
-      if (k) atomicAdd(&core_cnt[u], k);
+      if (k) atomicAdd(&J.core_cnt[u], k);
     }
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_label(const int32_t *__restrict__ core_cnt, uint32_t n, int32_t mu,
-                                                    int32_t *__restrict__ label, int32_t *__restrict__ parent,
-                                                    int32_t *__restrict__ rmin, int32_t *__restrict__ rmax,
-                                                    int32_t *__restrict__ cl) {
-  for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < n; u += gridDim.x * 256u) {
-    label[u] = core_cnt[u] >= mu ? 1 : 0;
-    parent[u] = static_cast<int32_t>(u);
-    rmin[u] = INT32_MAX;
-    rmax[u] = -1;
-    cl[u] = INT32_MAX;
+__global__ __launch_bounds__(256) void k_sj_label(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
+    J.label[u] = J.core_cnt[u] >= J.mu ? 1 : 0;
+    J.parent[u] = static_cast<int32_t>(u);
+    J.rmin[u] = INT32_MAX;
+    J.rmax[u] = -1;
+    J.cl[u] = INT32_MAX;
   }
 }
 
 // ClusterCore: union over similar core-core edges (each undirected edge once).
-__global__ __launch_bounds__(256) void k_scan_union(const int32_t *__restrict__ adj, const int32_t *__restrict__ src,
-                                                    const uint8_t *__restrict__ sim, const int32_t *__restrict__ label,
-                                                    uint64_t m, int32_t *parent) {
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += gridDim.x * 256ull) {
-    const int32_t u = src[i];
-    const int32_t v = adj[i];
-    if (u < v && sim[i] && label[u] == 1 && label[v] == 1) uf_union(parent, u, v);
+__global__ __launch_bounds__(256) void k_sj_union(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
+    const int32_t u = J.src[i];
+    const int32_t v = J.adj[i];
+    if (u < v && J.sim[i] && J.label[u] == 1 && J.label[v] == 1) uf_union(J.parent, u, v);
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_compress(const int32_t *__restrict__ label, uint32_t n,
-                                                       int32_t *parent) {
-  for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < n; u += gridDim.x * 256u)
-    if (label[u] == 1)
-      __hip_atomic_store(&parent[u], uf_find(parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
+__global__ __launch_bounds__(256) void k_sj_compress(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u)
+    if (J.label[u] == 1)
+      __hip_atomic_store(&J.parent[u], uf_find(J.parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // CheckHub: per non-core vertex, the smallest and largest root among its core neighbors.
-__global__ __launch_bounds__(256) void k_scan_hub(const int32_t *__restrict__ adj, const int32_t *__restrict__ src,
-                                                  const int32_t *__restrict__ label,
-                                                  const int32_t *__restrict__ parent, uint64_t m,
-                                                  int32_t *__restrict__ rmin, int32_t *__restrict__ rmax) {
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += gridDim.x * 256ull) {
-    const int32_t a = src[i];
-    const int32_t v = adj[i];
-    if (label[a] != 1 && label[v] == 1) {
-      const int32_t r = parent[v];
-      if (r < rmin[a]) atomicMin(&rmin[a], r);
-      if (r > rmax[a]) atomicMax(&rmax[a], r);
+__global__ __launch_bounds__(256) void k_sj_hub(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
+    const int32_t a = J.src[i];
+    const int32_t v = J.adj[i];
+    if (J.label[a] != 1 && J.label[v] == 1) {
+      const int32_t r = J.parent[v];
+      if (r < J.rmin[a]) atomicMin(&J.rmin[a], r);
+      if (r > J.rmax[a]) atomicMax(&J.rmax[a], r);
     }
   }
 }
 
 // Labels hubs, takes the minimum old id of every cluster and counts
 // cores, clusters, hubs and outliers (counters[0..3]).
-__global__ __launch_bounds__(256) void k_scan_finalize(uint32_t n, int32_t *__restrict__ label,
-                                                       const int32_t *__restrict__ parent,
-                                                       const int32_t *__restrict__ rmin,
-                                                       const int32_t *__restrict__ rmax,
-                                                       const int32_t *__restrict__ old_id, int32_t *__restrict__ cl,
-                                                       unsigned long long *__restrict__ counters) {
+__global__ __launch_bounds__(256) void k_sj_finalize(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
   unsigned long long cores = 0, clusters = 0, hubs = 0, outliers = 0;
-  for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < n; u += gridDim.x * 256u) {
-    if (label[u] == 1) {
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
+    if (J.label[u] == 1) {
       cores++;
-      const int32_t r = parent[u];
+      const int32_t r = J.parent[u];
       if (r == static_cast<int32_t>(u)) clusters++;
-      const int32_t o = old_id ? old_id[u] : static_cast<int32_t>(u);
-      atomicMin(&cl[r], o);
-    } else if (rmax[u] >= 0 && rmin[u] != rmax[u]) {
-      label[u] = 2;
+      const int32_t o = J.old ? J.old[u] : static_cast<int32_t>(u);
+      atomicMin(&J.cl[r], o);
+    } else if (J.rmax[u] >= 0 && J.rmin[u] != J.rmax[u]) {
+      J.label[u] = 2;
       hubs++;
     } else {
       outliers++;
@@ -202,36 +223,36 @@ __global__ __launch_bounds__(256) void k_scan_finalize(uint32_t n, int32_t *__re
   if (threadIdx.x < 4) {
     const unsigned long long t = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
                                  part[3][threadIdx.x];
-    if (t) atomicAdd(&counters[threadIdx.x], t);
+    if (t) atomicAdd(&J.counters[threadIdx.x], t);
   }
 }
 
-__global__ __launch_bounds__(256) void k_scan_cid(uint32_t n, const int32_t *__restrict__ label,
-                                                  const int32_t *__restrict__ parent,
-                                                  const int32_t *__restrict__ cl, int32_t *__restrict__ cid) {
-  for (uint32_t u = blockIdx.x * 256u + threadIdx.x; u < n; u += gridDim.x * 256u)
-    cid[u] = label[u] == 1 ? cl[parent[u]] : -1;
+__global__ __launch_bounds__(256) void k_sj_cid(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u)
+    J.cid[u] = J.label[u] == 1 ? J.cl[J.parent[u]] : -1;
+}
+
+__global__ __launch_bounds__(64) void k_sj_zero_pairs(const ScanJob *__restrict__ jobs, int nj) {
+  if (static_cast<int>(threadIdx.x) < nj) jobs[threadIdx.x].counters[4] = 0;
 }
 
 // Non-core memberships: (cluster id of core u, old id of v) for every similar
 // slot (u, v) with u core and v not core, emitted in any order; counters[4]
-// counts them, capacity cap.  Every lane takes its position from the same
-// counter with an increment of 0 or 1, so positions are unique without a
+// counts them, capacity pairs_cap.  Every lane takes its position from the
+// same counter with an increment of 0 or 1, so positions are unique without a
 // lane-0 broadcast.
-__global__ __launch_bounds__(256) void k_scan_pairs(const int32_t *__restrict__ adj, const int32_t *__restrict__ src,
-                                                    const uint8_t *__restrict__ sim,
-                                                    const int32_t *__restrict__ label,
-                                                    const int32_t *__restrict__ cid,
-                                                    const int32_t *__restrict__ old_id, uint64_t m,
-                                                    int2 *__restrict__ pairs, uint64_t cap,
-                                                    unsigned long long *__restrict__ counters) {
-  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += gridDim.x * 256ull) {
-    if (!sim[i]) continue;
-    const int32_t u = src[i];
-    const int32_t v = adj[i];
-    if (label[u] != 1 || label[v] == 1) continue;
-    const unsigned long long pos = atomicAdd(&counters[4], 1ull);
-    if (pos < cap) pairs[pos] = make_int2(cid[u], old_id ? old_id[v] : v);
+__global__ __launch_bounds__(256) void k_sj_pairs(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
+    if (!J.sim[i]) continue;
+    const int32_t u = J.src[i];
+    const int32_t v = J.adj[i];
+    if (J.label[u] != 1 || J.label[v] == 1) continue;
+    const unsigned long long pos = atomicAdd(&J.counters[4], 1ull);
+    if (pos < J.pairs_cap) J.pairs[pos] = make_int2(J.cid[u], J.old ? J.old[v] : v);
   }
 }
 
@@ -243,19 +264,55 @@ inline unsigned grid_for(uint64_t items, int cus) {
 
 }  // namespace
 
+namespace accscan {
+
+// Workgroup ranges of every job: a job gets one workgroup per 256 items, at
+// most 8 per CU, at least one.
+void plan_jobs(ScanJob *jobs, int nj, int cus, uint32_t *vblocks, uint32_t *sblocks) {
+  uint32_t v = 0, s = 0;
+  for (int j = 0; j < nj; j++) {
+    jobs[j].vb0 = v;
+    jobs[j].vbn = grid_for(jobs[j].n, cus);
+    jobs[j].sb0 = s;
+    jobs[j].sbn = grid_for(jobs[j].m, cus);
+    v += jobs[j].vbn;
+    s += jobs[j].sbn;
+  }
+  *vblocks = v;
+  *sblocks = s;
+}
+
+int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sj_init, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_sim, dim3(sblocks), dim3(kScanWG), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_label, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_union, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_compress, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_hub, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_pairs(const ScanJob *d_jobs, int nj, uint32_t sblocks, hipStream_t s) {
+  if (nj > 64) return acc_set_error(ACC_EINVAL, "at most 64 jobs per pairs launch");
+  hipLaunchKernelGGL(k_sj_zero_pairs, dim3(1), dim3(64), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_pairs, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+}  // namespace accscan
+
 struct acc_scan_ws {
   acc_graph *g = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;  // counts of the graph are complete
-  uint8_t *d_sim = nullptr;
-  int32_t *d_core_cnt = nullptr;
-  int32_t *d_label = nullptr;
-  int32_t *d_parent = nullptr;
-  int32_t *d_rmin = nullptr;
-  int32_t *d_rmax = nullptr;
-  int32_t *d_cl = nullptr;
-  int32_t *d_cid = nullptr;
-  unsigned long long *d_counters = nullptr;  // core, cluster, hub, outlier, pairs
+  ScanBuffers buf;
+  ScanJob *d_job = nullptr;
+  ScanJob *h_job = nullptr;  // pinned staging of the job description
+  uint32_t vblocks = 0, sblocks = 0;
   int2 *d_pairs = nullptr;
   uint64_t pairs_cap = 0;
   uint64_t device_bytes = 0;
@@ -268,29 +325,100 @@ void free_ws(acc_scan_ws *w) {
   if (!w) return;
   (void)hipSetDevice(w->g->ctx->device);
   if (w->stream) (void)hipStreamSynchronize(w->stream);
-  for (void *p : {static_cast<void *>(w->d_sim), static_cast<void *>(w->d_core_cnt),
-                  static_cast<void *>(w->d_label), static_cast<void *>(w->d_parent), static_cast<void *>(w->d_rmin),
-                  static_cast<void *>(w->d_rmax), static_cast<void *>(w->d_cl), static_cast<void *>(w->d_cid),
-                  static_cast<void *>(w->d_counters), static_cast<void *>(w->d_pairs)})
-    if (p) (void)hipFree(p);
+  accscan::free_buffers(&w->buf);
+  if (w->d_job) (void)hipFree(w->d_job);
+  if (w->h_job) (void)hipHostFree(w->h_job);
+  if (w->d_pairs) (void)hipFree(w->d_pairs);
   if (w->ready) (void)hipEventDestroy(w->ready);
   if (w->stream) (void)hipStreamDestroy(w->stream);
   delete w;
 }
 
-template <typename T>
-int ws_alloc(acc_scan_ws *w, T **p, uint64_t count) {
-  const size_t bytes = sizeof(T) * (count ? count : 1);
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return acc_set_error(ACC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+}  // namespace
+
+namespace accscan {
+
+void free_buffers(ScanBuffers *b) {
+  for (void *p : {static_cast<void *>(b->sim), static_cast<void *>(b->core_cnt), static_cast<void *>(b->label),
+                  static_cast<void *>(b->parent), static_cast<void *>(b->rmin), static_cast<void *>(b->rmax),
+                  static_cast<void *>(b->cl), static_cast<void *>(b->cid), static_cast<void *>(b->counters)})
+    if (p) (void)hipFree(p);
+  *b = ScanBuffers{};
+}
+
+// Grows the per-query buffers to hold a graph of n vertices and m slots.
+int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m) {
+  if (b->counters && n <= b->n_cap && m <= b->m_cap) return ACC_OK;
+  const uint32_t nc = std::max(n, b->n_cap);
+  const uint64_t mc = std::max(m, b->m_cap);
+  free_buffers(b);
+  auto alloc = [&](auto **p, uint64_t count) -> int {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(*p)>>;
+    const size_t bytes = sizeof(T) * (count ? count : 1);
+    if (hipMalloc(reinterpret_cast<void **>(p), bytes) != hipSuccess) {
+      *p = nullptr;
+      return acc_set_error(ACC_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes of SCAN buffers failed");
+    }
+    b->bytes += bytes;
+    return ACC_OK;
+  };
+  int rc = ACC_OK;
+  if (!rc) rc = alloc(&b->sim, mc);
+  if (!rc) rc = alloc(&b->core_cnt, nc);
+  if (!rc) rc = alloc(&b->label, nc);
+  if (!rc) rc = alloc(&b->parent, nc);
+  if (!rc) rc = alloc(&b->rmin, nc);
+  if (!rc) rc = alloc(&b->rmax, nc);
+  if (!rc) rc = alloc(&b->cl, nc);
+  if (!rc) rc = alloc(&b->cid, nc);
+  if (!rc) rc = alloc(&b->counters, 8);
+  if (rc) {
+    free_buffers(b);
+    return rc;
   }
-  w->device_bytes += bytes;
+  b->n_cap = nc;
+  b->m_cap = mc;
   return ACC_OK;
 }
 
-}  // namespace
+uint64_t buffer_bytes(uint32_t n, uint64_t m) {
+  const uint64_t nn = n ? n : 1, mm = m ? m : 1;
+  return mm + 8 * sizeof(int32_t) * nn + 8 * sizeof(unsigned long long);
+}
+
+int check_eps(int64_t eps_num, int64_t eps_den) {
+  if (eps_num < 0 || eps_den <= 0 || eps_num > (1ll << 31) || eps_den > (1ll << 31))
+    return acc_set_error(ACC_EINVAL, "eps must be eps_num/eps_den with 0 <= eps_num and 0 < eps_den <= 2^31");
+  return ACC_OK;
+}
+
+// The job of one query of graph g with the given buffers.
+ScanJob make_job(const acc_graph *g, const ScanBuffers &b, int64_t eps_num, int64_t eps_den, int32_t mu) {
+  ScanJob j{};
+  j.off = g->d_off;
+  j.adj = g->d_adj;
+  j.src = g->d_src;
+  j.cnt = g->d_cnt;
+  j.old = g->d_old;
+  j.sim = b.sim;
+  j.core_cnt = b.core_cnt;
+  j.label = b.label;
+  j.parent = b.parent;
+  j.rmin = b.rmin;
+  j.rmax = b.rmax;
+  j.cl = b.cl;
+  j.cid = b.cid;
+  j.counters = b.counters;
+  j.p2 = static_cast<unsigned long long>(eps_num) * static_cast<unsigned long long>(eps_num);
+  j.q2 = static_cast<unsigned long long>(eps_den) * static_cast<unsigned long long>(eps_den);
+  j.m = g->m;
+  j.n = g->n;
+  j.add = g->cnt_plus2 ? 0 : 2;
+  j.mu = mu;
+  return j;
+}
+
+}  // namespace accscan
 
 // Builds the slot -> source array of a resident graph once.
 int acc_graph_ensure_src(acc_graph *g) {
@@ -348,15 +476,16 @@ int acc_scan_ws_create(acc_graph *g, acc_scan_ws **out) {
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e)));
-  if ((rc = ws_alloc(w, &w->d_sim, g->m))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_core_cnt, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_label, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_parent, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_rmin, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_rmax, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_cl, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_cid, g->n))) return bail(rc);
-  if ((rc = ws_alloc(w, &w->d_counters, 8))) return bail(rc);
+  if ((rc = accscan::reserve_buffers(&w->buf, g->n, g->m))) return bail(rc);
+  if (hipMalloc(reinterpret_cast<void **>(&w->d_job), sizeof(ScanJob)) != hipSuccess) {
+    w->d_job = nullptr;
+    return bail(acc_set_error(ACC_ENOMEM, "hipMalloc of the SCAN job failed"));
+  }
+  if (hipHostMalloc(reinterpret_cast<void **>(&w->h_job), sizeof(ScanJob), hipHostMallocDefault) != hipSuccess) {
+    w->h_job = nullptr;
+    return bail(acc_set_error(ACC_ENOMEM, "hipHostMalloc of the SCAN job failed"));
+  }
+  w->device_bytes = w->buf.bytes + sizeof(ScanJob);
   if ((rc = acc_graph_ensure_src(g))) return bail(rc);
   *out = w;
   return ACC_OK;
@@ -373,34 +502,19 @@ int acc_scan_ws_info(const acc_scan_ws *w, uint64_t *device_bytes, void **stream
 
 int acc_scan_run(acc_scan_ws *w, int64_t eps_num, int64_t eps_den, int32_t mu, uint32_t flags) {
   if (!w) return acc_set_error(ACC_EINVAL, "null workspace");
-  if (eps_num < 0 || eps_den <= 0 || eps_num > (1ll << 31) || eps_den > (1ll << 31))
-    return acc_set_error(ACC_EINVAL, "eps must be eps_num/eps_den with 0 <= eps_num and 0 < eps_den <= 2^31");
+  ACC_TRY(accscan::check_eps(eps_num, eps_den));
   acc_graph *g = w->g;
   (void)hipSetDevice(g->ctx->device);
   if (!g->have_result) ACC_TRY(acc_cn_compute(g, ACC_PATH_AUTO, 0));
   // order this query after the graph's counts
   ACC_HIP_TRY(hipEventRecord(w->ready, g->ctx->stream));
   ACC_HIP_TRY(hipStreamWaitEvent(w->stream, w->ready, 0));
-  const int cus = g->ctx->num_cus;
-  hipStream_t s = w->stream;
-  const unsigned long long p2 = static_cast<unsigned long long>(eps_num) * static_cast<unsigned long long>(eps_num);
-  const unsigned long long q2 = static_cast<unsigned long long>(eps_den) * static_cast<unsigned long long>(eps_den);
-  const int32_t add = g->cnt_plus2 ? 0 : 2;
-  hipLaunchKernelGGL(k_scan_init, dim3(grid_for(g->n, cus)), dim3(256), 0, s, w->d_core_cnt, g->n, w->d_counters);
-  hipLaunchKernelGGL(k_scan_sim, dim3(grid_for(g->m, cus)), dim3(kScanWG), 0, s, g->d_off, g->d_adj, g->d_src,
-                     g->d_cnt, g->m, add, p2, q2, w->d_sim, w->d_core_cnt);
-  hipLaunchKernelGGL(k_scan_label, dim3(grid_for(g->n, cus)), dim3(256), 0, s, w->d_core_cnt, g->n, mu, w->d_label,
-                     w->d_parent, w->d_rmin, w->d_rmax, w->d_cl);
-  hipLaunchKernelGGL(k_scan_union, dim3(grid_for(g->m, cus)), dim3(256), 0, s, g->d_adj, g->d_src, w->d_sim,
-                     w->d_label, g->m, w->d_parent);
-  hipLaunchKernelGGL(k_scan_compress, dim3(grid_for(g->n, cus)), dim3(256), 0, s, w->d_label, g->n, w->d_parent);
-  hipLaunchKernelGGL(k_scan_hub, dim3(grid_for(g->m, cus)), dim3(256), 0, s, g->d_adj, g->d_src, w->d_label,
-                     w->d_parent, g->m, w->d_rmin, w->d_rmax);
-  hipLaunchKernelGGL(k_scan_finalize, dim3(grid_for(g->n, cus)), dim3(256), 0, s, g->n, w->d_label, w->d_parent,
-                     w->d_rmin, w->d_rmax, g->d_old, w->d_cl, w->d_counters);
-  hipLaunchKernelGGL(k_scan_cid, dim3(grid_for(g->n, cus)), dim3(256), 0, s, g->n, w->d_label, w->d_parent, w->d_cl,
-                     w->d_cid);
-  ACC_HIP_TRY(hipGetLastError());
+  // the staging of the previous query's job may still be in flight
+  ACC_HIP_TRY(hipStreamSynchronize(w->stream));
+  *w->h_job = accscan::make_job(g, w->buf, eps_num, eps_den, mu);
+  accscan::plan_jobs(w->h_job, 1, g->ctx->num_cus, &w->vblocks, &w->sblocks);
+  ACC_HIP_TRY(hipMemcpyAsync(w->d_job, w->h_job, sizeof(ScanJob), hipMemcpyHostToDevice, w->stream));
+  ACC_TRY(accscan::launch_jobs(w->d_job, 1, w->vblocks, w->sblocks, w->stream));
   (void)flags;
   w->have_result = true;
   return ACC_OK;
@@ -420,13 +534,14 @@ int acc_scan_results(acc_scan_ws *w, int32_t *label, int32_t *cluster_id, uint8_
   acc_graph *g = w->g;
   (void)hipSetDevice(g->ctx->device);
   hipStream_t s = w->stream;
-  if (label) ACC_HIP_TRY(hipMemcpyAsync(label, w->d_label, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
-  if (cluster_id) ACC_HIP_TRY(hipMemcpyAsync(cluster_id, w->d_cid, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
-  if (sim && g->m) ACC_HIP_TRY(hipMemcpyAsync(sim, w->d_sim, g->m, hipMemcpyDeviceToHost, s));
+  const ScanBuffers &b = w->buf;
+  if (label) ACC_HIP_TRY(hipMemcpyAsync(label, b.label, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
+  if (cluster_id) ACC_HIP_TRY(hipMemcpyAsync(cluster_id, b.cid, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
+  if (sim && g->m) ACC_HIP_TRY(hipMemcpyAsync(sim, b.sim, g->m, hipMemcpyDeviceToHost, s));
   if (core_count)
-    ACC_HIP_TRY(hipMemcpyAsync(core_count, w->d_core_cnt, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
+    ACC_HIP_TRY(hipMemcpyAsync(core_count, b.core_cnt, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, s));
   unsigned long long c[8] = {0};
-  ACC_HIP_TRY(hipMemcpyAsync(c, w->d_counters, sizeof(c), hipMemcpyDeviceToHost, s));
+  ACC_HIP_TRY(hipMemcpyAsync(c, b.counters, sizeof(c), hipMemcpyDeviceToHost, s));
   ACC_HIP_TRY(hipStreamSynchronize(s));
   if (counts)
     for (int i = 0; i < 4; i++) counts[i] = static_cast<int64_t>(c[i]);
@@ -441,16 +556,21 @@ int acc_scan_pairs(acc_scan_ws *w, int32_t *pairs, uint64_t cap, uint64_t *count
   hipStream_t s = w->stream;
   // every slot is at most one pair: size the buffer for m once
   if (!w->d_pairs) {
-    int rc = ws_alloc(w, &w->d_pairs, g->m);
-    if (rc) return rc;
+    const size_t bytes = sizeof(int2) * (g->m ? g->m : 1);
+    if (hipMalloc(reinterpret_cast<void **>(&w->d_pairs), bytes) != hipSuccess) {
+      w->d_pairs = nullptr;
+      return acc_set_error(ACC_ENOMEM, "hipMalloc of the pair buffer failed");
+    }
+    w->device_bytes += bytes;
     w->pairs_cap = g->m;
   }
-  ACC_HIP_TRY(hipMemsetAsync(w->d_counters + 4, 0, sizeof(unsigned long long), s));
-  hipLaunchKernelGGL(k_scan_pairs, dim3(grid_for(g->m, g->ctx->num_cus)), dim3(256), 0, s, g->d_adj, g->d_src,
-                     w->d_sim, w->d_label, w->d_cid, g->d_old, g->m, w->d_pairs, w->pairs_cap, w->d_counters);
-  ACC_HIP_TRY(hipGetLastError());
+  ACC_HIP_TRY(hipStreamSynchronize(s));
+  w->h_job->pairs = w->d_pairs;
+  w->h_job->pairs_cap = w->pairs_cap;
+  ACC_HIP_TRY(hipMemcpyAsync(w->d_job, w->h_job, sizeof(ScanJob), hipMemcpyHostToDevice, s));
+  ACC_TRY(accscan::launch_pairs(w->d_job, 1, w->sblocks, s));
   unsigned long long k = 0;
-  ACC_HIP_TRY(hipMemcpyAsync(&k, w->d_counters + 4, sizeof(k), hipMemcpyDeviceToHost, s));
+  ACC_HIP_TRY(hipMemcpyAsync(&k, w->buf.counters + 4, sizeof(k), hipMemcpyDeviceToHost, s));
   ACC_HIP_TRY(hipStreamSynchronize(s));
   *count = k;
   if (pairs && k) {

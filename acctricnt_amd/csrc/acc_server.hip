@@ -1,0 +1,618 @@
+// Serving loop for SCAN queries over resident graphs (include/acc_cn.h,
+// "Serving").
+//
+// Graphs are registered with a host copy of their CSR and kept resident in
+// HBM, with their common-neighbor counts computed once at admission, while
+// their device footprint fits the configured budget; admitting a graph that
+// does not fit evicts the least recently used resident graphs that have no
+// query running (they are admitted again, from the host copy, when a query
+// needs them).  Queries are (graph, eps = p/q, mu) triples.  A worker thread
+// forms batches of pending queries across graphs and runs each batch as one
+// set of SCAN launches (one job table, acc_scan.hip) on one of several
+// streams, so a batch is launched while earlier batches are still on the
+// device: arrivals are admitted continuously as result slots and streams
+// free up.  Each query owns a result slot (device buffers plus pinned host
+// copies of its labels and cluster ids) from launch until the caller
+// releases it.  Latency runs from submission to the worker observing the
+// batch's completion event, with the results already in host memory.
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "acc_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct ServedGraph {
+  std::vector<uint32_t> off;
+  std::vector<int32_t> adj;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  acc_graph *g = nullptr;  // resident copy, or NULL when evicted
+  uint64_t bytes = 0;      // device bytes when resident (estimate until uploaded)
+  uint64_t last_use = 0;   // LRU clock
+  int running = 0;         // queries launched and not complete
+  int waiting = 0;         // queries pending
+  bool validated = false;  // the host copy passed the upload checks once
+};
+
+enum QueryState { kPending = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+
+struct Query {
+  int32_t gid = -1;
+  int64_t p = 0, q = 1;
+  int32_t mu = 0;
+  Clock::time_point t_submit, t_done;
+  int slot = -1;
+  int state = kPending;
+  int err = ACC_OK;
+  std::string err_text;
+  int64_t counts[4] = {0, 0, 0, 0};
+};
+
+struct Slot {
+  ScanBuffers buf;
+  int32_t *h_label = nullptr;  // pinned
+  int32_t *h_cid = nullptr;    // pinned
+  unsigned long long *h_counters = nullptr;
+  uint32_t h_cap = 0;
+  bool busy = false;
+};
+
+struct Batch {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  ScanJob *d_jobs = nullptr;
+  ScanJob *h_jobs = nullptr;  // pinned staging
+  std::vector<uint64_t> tickets;
+  bool busy = false;
+};
+
+uint64_t graph_estimate(uint32_t n, uint64_t m) {
+  // offsets, adjacency, counts, reverse slots, slot sources, work plans
+  return 4ull * (n + 1) + 4ull * (m + 64) + 4ull * (m + 4) * 2 + 4ull * m + 16ull * (n + m / 64 + 16) + (1ull << 20);
+}
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace
+
+struct acc_server {
+  acc_ctx *ctx = nullptr;
+  acc_server_config cfg{};
+  std::mutex mu;
+  std::mutex admit_mu;  // serialises uploads on the context stream
+  std::condition_variable cv_work, cv_done;
+  std::map<int32_t, ServedGraph> graphs;
+  int32_t next_gid = 0;
+  std::deque<uint64_t> pending;
+  std::unordered_map<uint64_t, Query> queries;
+  uint64_t next_ticket = 1;
+  std::vector<Slot> slots;
+  std::vector<Batch> batches;
+  std::thread worker;
+  bool stop = false;
+  uint64_t clock = 0;
+  uint64_t graph_bytes = 0;  // resident graphs
+  uint64_t slot_bytes = 0;   // result slots
+  acc_server_stats st{};
+};
+
+namespace {
+
+void free_slot(Slot &s) {
+  accscan::free_buffers(&s.buf);
+  if (s.h_label) (void)hipHostFree(s.h_label);
+  if (s.h_cid) (void)hipHostFree(s.h_cid);
+  if (s.h_counters) (void)hipHostFree(s.h_counters);
+  s = Slot{};
+}
+
+// Device bytes the budget still has room for.
+uint64_t room(const acc_server *s) {
+  const uint64_t used = s->graph_bytes + s->slot_bytes;
+  return used >= s->cfg.hbm_budget ? 0 : s->cfg.hbm_budget - used;
+}
+
+// Evicts least recently used resident graphs with no running or pending
+// query (other than `keep`) until `need` bytes fit.  Caller holds s->mu.
+bool make_room(acc_server *s, uint64_t need, int32_t keep) {
+  while (room(s) < need) {
+    ServedGraph *victim = nullptr;
+    for (auto &kv : s->graphs) {
+      ServedGraph &sg = kv.second;
+      if (kv.first == keep || !sg.g || sg.running || sg.waiting) continue;
+      if (!victim || sg.last_use < victim->last_use) victim = &sg;
+    }
+    if (!victim) return false;
+    acc_graph_release(victim->g);
+    victim->g = nullptr;
+    s->graph_bytes -= victim->bytes;
+    s->st.evictions++;
+  }
+  return true;
+}
+
+// Uploads graph gid and computes its counts.  Called without s->mu held
+// (the upload takes milliseconds); s->admit_mu serialises admissions.
+int upload_graph(acc_server *s, ServedGraph &sg) {
+  acc_graph *g = nullptr;
+  int rc = acc_graph_upload(s->ctx, sg.n, sg.m, sg.off.data(), sg.adj.data(),
+                            sg.validated ? ACC_UPLOAD_NO_VALIDATE : 0u, &g);
+  if (!rc) rc = acc_cn_compute(g, ACC_PATH_AUTO, 0);
+  if (!rc) rc = acc_graph_ensure_src(g);
+  if (!rc) rc = acc_ctx_synchronize(s->ctx);
+  if (rc) {
+    if (g) acc_graph_release(g);
+    return rc;
+  }
+  sg.g = g;
+  sg.validated = true;
+  return ACC_OK;
+}
+
+// Makes gid resident: reserves its footprint under the lock, uploads without
+// it.  Caller holds `lk` on s->mu; it is released during the upload.
+int admit(acc_server *s, std::unique_lock<std::mutex> &lk, int32_t gid) {
+  auto it = s->graphs.find(gid);
+  if (it == s->graphs.end()) return acc_set_error(ACC_ENOTFOUND, "unknown graph id " + std::to_string(gid));
+  ServedGraph &sg = it->second;
+  if (sg.g) return ACC_OK;
+  const uint64_t need = graph_estimate(sg.n, sg.m);
+  if (need > s->cfg.hbm_budget) return acc_set_error(ACC_EBUDGET, "graph exceeds the HBM budget");
+  if (!make_room(s, need, gid)) return acc_set_error(ACC_EBUDGET, "no idle graph left to evict");
+  s->graph_bytes += need;
+  sg.bytes = need;
+  sg.running++;  // pins the entry against eviction and removal while unlocked
+  lk.unlock();
+  int rc;
+  {
+    std::lock_guard<std::mutex> ag(s->admit_mu);
+    rc = upload_graph(s, sg);
+  }
+  lk.lock();
+  sg.running--;
+  s->graph_bytes -= need;
+  if (rc) {
+    sg.bytes = 0;
+    return rc;
+  }
+  uint64_t actual = 0;
+  (void)acc_graph_info(sg.g, nullptr, nullptr, &actual);
+  sg.bytes = actual;
+  s->graph_bytes += actual;
+  s->st.admissions++;
+  return ACC_OK;
+}
+
+// Grows slot k's buffers for a graph of n vertices and m slots.  Caller holds s->mu.
+int reserve_slot(acc_server *s, Slot &sl, uint32_t n, uint64_t m) {
+  const uint64_t before = sl.buf.bytes;
+  if (!(sl.buf.counters && n <= sl.buf.n_cap && m <= sl.buf.m_cap)) {
+    const uint64_t grow = accscan::buffer_bytes(std::max(n, sl.buf.n_cap), std::max(m, sl.buf.m_cap));
+    if (grow > before && !make_room(s, grow - before, -1))
+      return acc_set_error(ACC_EBUDGET, "no room for the result buffers of a query");
+    ACC_TRY(accscan::reserve_buffers(&sl.buf, n, m));
+    s->slot_bytes += sl.buf.bytes - before;
+  }
+  if (n > sl.h_cap || !sl.h_counters) {
+    if (sl.h_label) (void)hipHostFree(sl.h_label);
+    if (sl.h_cid) (void)hipHostFree(sl.h_cid);
+    sl.h_label = sl.h_cid = nullptr;
+    sl.h_cap = 0;
+    const size_t bytes = sizeof(int32_t) * std::max<uint32_t>(n, 1);
+    if (hipHostMalloc(reinterpret_cast<void **>(&sl.h_label), bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&sl.h_cid), bytes, hipHostMallocDefault) != hipSuccess)
+      return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result buffers failed");
+    if (!sl.h_counters &&
+        hipHostMalloc(reinterpret_cast<void **>(&sl.h_counters), 8 * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess)
+      return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result counters failed");
+    sl.h_cap = std::max<uint32_t>(n, 1);
+  }
+  return ACC_OK;
+}
+
+void fail_query(acc_server *s, Query &q, int rc) {
+  q.state = kFailed;
+  q.err = rc;
+  q.err_text = acc_last_error();
+  q.t_done = Clock::now();
+  s->st.failed++;
+}
+
+// Marks finished batches' queries done.  Caller holds s->mu.
+bool reap(acc_server *s) {
+  bool any = false;
+  for (Batch &b : s->batches) {
+    if (!b.busy) continue;
+    const hipError_t e = hipEventQuery(b.done);
+    if (e == hipErrorNotReady) continue;
+    const Clock::time_point now = Clock::now();
+    for (uint64_t t : b.tickets) {
+      auto it = s->queries.find(t);
+      if (it == s->queries.end()) continue;
+      Query &q = it->second;
+      ServedGraph &sg = s->graphs[q.gid];
+      sg.running--;
+      if (e != hipSuccess) {
+        acc_set_error(ACC_EDEVICE, std::string("SCAN batch: ") + hipGetErrorString(e));
+        fail_query(s, q, ACC_EDEVICE);
+        continue;
+      }
+      const Slot &sl = s->slots[q.slot];
+      for (int i = 0; i < 4; i++) q.counts[i] = static_cast<int64_t>(sl.h_counters[i]);
+      q.state = kDone;
+      q.t_done = now;
+      s->st.completed++;
+    }
+    b.tickets.clear();
+    b.busy = false;
+    any = true;
+  }
+  return any;
+}
+
+bool any_running(const acc_server *s) {
+  for (const Batch &b : s->batches)
+    if (b.busy) return true;
+  return false;
+}
+
+// Admits the graph of the first pending query whose graph is not resident.
+// Returns true when it did something (the caller rescans the queue, which
+// may have changed while the lock was released).  Caller holds `lk`.
+bool admit_pending(acc_server *s, std::unique_lock<std::mutex> &lk) {
+  for (size_t i = 0; i < s->pending.size(); i++) {
+    const uint64_t t = s->pending[i];
+    Query &q = s->queries[t];
+    if (s->graphs[q.gid].g) continue;
+    const int rc = admit(s, lk, q.gid);
+    if (rc == ACC_EBUDGET && any_running(s)) return false;  // retry once running queries free graphs
+    if (rc) {
+      auto qi = s->queries.find(t);
+      if (qi != s->queries.end() && qi->second.state == kPending) {
+        s->graphs[qi->second.gid].waiting--;
+        fail_query(s, qi->second, rc);
+        auto pi = std::find(s->pending.begin(), s->pending.end(), t);
+        if (pi != s->pending.end()) s->pending.erase(pi);
+        s->cv_done.notify_all();
+      }
+    }
+    return true;
+  }
+  return false;
+}
+
+// Launches one batch of pending queries whose graphs are resident, if a
+// stream and result slots are free.  Caller holds s->mu.
+bool launch_batch(acc_server *s) {
+  Batch *b = nullptr;
+  for (Batch &x : s->batches)
+    if (!x.busy) {
+      b = &x;
+      break;
+    }
+  if (!b || s->pending.empty()) return false;
+  std::vector<uint64_t> take;
+  std::vector<int> slot_of;
+  const size_t cap = static_cast<size_t>(s->cfg.max_batch);
+  size_t next_slot = 0;
+  for (size_t i = 0; i < s->pending.size() && take.size() < cap; i++) {
+    const uint64_t t = s->pending[i];
+    Query &q = s->queries[t];
+    ServedGraph &sg = s->graphs[q.gid];
+    if (!sg.g) continue;
+    while (next_slot < s->slots.size() && s->slots[next_slot].busy) next_slot++;
+    if (next_slot == s->slots.size()) break;  // every slot holds a result not yet released
+    const int rc = reserve_slot(s, s->slots[next_slot], sg.n, sg.m);
+    if (rc == ACC_EBUDGET && (any_running(s) || !take.empty())) break;
+    if (rc) {
+      sg.waiting--;
+      fail_query(s, q, rc);
+      s->pending[i] = 0;  // erased below
+      s->cv_done.notify_all();
+      continue;
+    }
+    s->slots[next_slot].busy = true;
+    take.push_back(t);
+    slot_of.push_back(static_cast<int>(next_slot));
+    s->pending[i] = 0;
+  }
+  s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), 0ull), s->pending.end());
+  if (take.empty()) return false;
+  const int nj = static_cast<int>(take.size());
+  for (int j = 0; j < nj; j++) {
+    Query &q = s->queries[take[j]];
+    ServedGraph &sg = s->graphs[q.gid];
+    sg.waiting--;
+    sg.running++;
+    sg.last_use = ++s->clock;
+    q.slot = slot_of[j];
+    q.state = kRunning;
+    b->h_jobs[j] = accscan::make_job(sg.g, s->slots[q.slot].buf, q.p, q.q, q.mu);
+  }
+  uint32_t vblocks = 0, sblocks = 0;
+  accscan::plan_jobs(b->h_jobs, nj, s->ctx->num_cus, &vblocks, &sblocks);
+  hipError_t e = hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(ScanJob) * nj, hipMemcpyHostToDevice, b->stream);
+  int rc = e == hipSuccess ? accscan::launch_jobs(b->d_jobs, nj, vblocks, sblocks, b->stream)
+                           : acc_set_error(ACC_EDEVICE, std::string("job upload: ") + hipGetErrorString(e));
+  for (int j = 0; j < nj && !rc; j++) {
+    const Query &q = s->queries[take[j]];
+    const ServedGraph &sg = s->graphs[q.gid];
+    const Slot &sl = s->slots[q.slot];
+    e = hipMemcpyAsync(sl.h_label, sl.buf.label, sizeof(int32_t) * sg.n, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.h_cid, sl.buf.cid, sizeof(int32_t) * sg.n, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.h_counters, sl.buf.counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         b->stream);
+    if (e != hipSuccess) rc = acc_set_error(ACC_EDEVICE, std::string("result copy: ") + hipGetErrorString(e));
+  }
+  if (!rc && (e = hipEventRecord(b->done, b->stream)) != hipSuccess)
+    rc = acc_set_error(ACC_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
+  if (rc) {
+    (void)hipStreamSynchronize(b->stream);
+    for (uint64_t t : take) {
+      Query &q = s->queries[t];
+      s->graphs[q.gid].running--;
+      fail_query(s, q, rc);
+    }
+    s->cv_done.notify_all();
+    return true;
+  }
+  b->busy = true;
+  b->tickets = take;
+  s->st.batches++;
+  s->st.batched_queries += static_cast<uint64_t>(nj);
+  s->st.max_batch_seen = std::max<uint64_t>(s->st.max_batch_seen, static_cast<uint64_t>(nj));
+  return true;
+}
+
+void worker_main(acc_server *s) {
+  (void)hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  while (!s->stop) {
+    if (reap(s)) s->cv_done.notify_all();
+    bool launched = false;
+    while (admit_pending(s, lk)) launched = true;
+    while (launch_batch(s)) launched = true;
+    bool inflight = false;
+    for (const Batch &b : s->batches) inflight = inflight || b.busy;
+    if (launched) continue;
+    if (inflight)
+      s->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    else
+      s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+  }
+}
+
+void destroy_server(acc_server *s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->ctx->device);
+  for (Batch &b : s->batches) {
+    if (b.stream) (void)hipStreamSynchronize(b.stream);
+    if (b.d_jobs) (void)hipFree(b.d_jobs);
+    if (b.h_jobs) (void)hipHostFree(b.h_jobs);
+    if (b.done) (void)hipEventDestroy(b.done);
+    if (b.stream) (void)hipStreamDestroy(b.stream);
+  }
+  for (Slot &sl : s->slots) free_slot(sl);
+  for (auto &kv : s->graphs)
+    if (kv.second.g) acc_graph_release(kv.second.g);
+  delete s;
+}
+
+}  // namespace
+
+extern "C" {
+
+void acc_server_config_default(acc_server_config *cfg) {
+  if (!cfg) return;
+  cfg->hbm_budget = 64ull << 30;
+  cfg->max_batch = 16;
+  cfg->streams = 4;
+  cfg->slots = 32;
+}
+
+int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **out) {
+  if (!ctx || !out) return acc_set_error(ACC_EINVAL, "null argument");
+  *out = nullptr;
+  acc_server_config c;
+  acc_server_config_default(&c);
+  if (cfg) c = *cfg;
+  if (c.max_batch < 1 || c.max_batch > 256 || c.streams < 1 || c.streams > 16 || c.slots < 1 || c.slots > 4096)
+    return acc_set_error(ACC_EINVAL, "server config: max_batch in [1,256], streams in [1,16], slots in [1,4096]");
+  (void)hipSetDevice(ctx->device);
+  auto *s = new (std::nothrow) acc_server;
+  if (!s) return acc_set_error(ACC_ENOMEM, "host allocation failed");
+  s->ctx = ctx;
+  s->cfg = c;
+  s->slots.resize(static_cast<size_t>(c.slots));
+  s->batches.resize(static_cast<size_t>(c.streams));
+  for (Batch &b : s->batches) {
+    hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&b.d_jobs), sizeof(ScanJob) * c.max_batch);
+    if (e == hipSuccess)
+      e = hipHostMalloc(reinterpret_cast<void **>(&b.h_jobs), sizeof(ScanJob) * c.max_batch, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      destroy_server(s);
+      return acc_set_error(ACC_EDEVICE, std::string("server setup: ") + hipGetErrorString(e));
+    }
+  }
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return ACC_OK;
+}
+
+void acc_server_destroy(acc_server *s) { destroy_server(s); }
+
+int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
+                         uint32_t flags, int32_t *graph_id) {
+  if (!s || !node_off || (m && !edge_dst) || !graph_id) return acc_set_error(ACC_EINVAL, "null argument");
+  if (node_off[0] != 0 || node_off[n] != m) return acc_set_error(ACC_EFORMAT, "node_off must run from 0 to m");
+  std::unique_lock<std::mutex> lk(s->mu);
+  const int32_t gid = s->next_gid++;
+  ServedGraph &sg = s->graphs[gid];
+  sg.n = n;
+  sg.m = m;
+  sg.off.assign(node_off, node_off + static_cast<size_t>(n) + 1);
+  sg.adj.assign(edge_dst, edge_dst + m);
+  sg.validated = (flags & ACC_UPLOAD_NO_VALIDATE) != 0;
+  const int rc = admit(s, lk, gid);
+  if (rc) {
+    s->graphs.erase(gid);
+    return rc;
+  }
+  s->graphs[gid].last_use = ++s->clock;
+  *graph_id = gid;
+  return ACC_OK;
+}
+
+int acc_server_remove_graph(acc_server *s, int32_t graph_id) {
+  if (!s) return acc_set_error(ACC_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->graphs.find(graph_id);
+  if (it == s->graphs.end()) return acc_set_error(ACC_ENOTFOUND, "unknown graph id " + std::to_string(graph_id));
+  if (it->second.running || it->second.waiting) return acc_set_error(ACC_EINVAL, "graph has queries in flight");
+  if (it->second.g) {
+    acc_graph_release(it->second.g);
+    s->graph_bytes -= it->second.bytes;
+  }
+  s->graphs.erase(it);
+  return ACC_OK;
+}
+
+int acc_server_graph_info(acc_server *s, int32_t graph_id, uint32_t *n, uint64_t *m, int *resident,
+                          uint64_t *device_bytes) {
+  if (!s) return acc_set_error(ACC_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->graphs.find(graph_id);
+  if (it == s->graphs.end()) return acc_set_error(ACC_ENOTFOUND, "unknown graph id " + std::to_string(graph_id));
+  if (n) *n = it->second.n;
+  if (m) *m = it->second.m;
+  if (resident) *resident = it->second.g != nullptr;
+  if (device_bytes) *device_bytes = it->second.g ? it->second.bytes : 0;
+  return ACC_OK;
+}
+
+int acc_server_submit(acc_server *s, int32_t graph_id, int64_t eps_num, int64_t eps_den, int32_t mu,
+                      uint64_t *ticket) {
+  if (!s || !ticket) return acc_set_error(ACC_EINVAL, "null argument");
+  ACC_TRY(accscan::check_eps(eps_num, eps_den));
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->graphs.find(graph_id);
+  if (it == s->graphs.end()) return acc_set_error(ACC_ENOTFOUND, "unknown graph id " + std::to_string(graph_id));
+  const uint64_t t = s->next_ticket++;
+  Query &q = s->queries[t];
+  q.gid = graph_id;
+  q.p = eps_num;
+  q.q = eps_den;
+  q.mu = mu;
+  q.t_submit = Clock::now();
+  it->second.waiting++;
+  s->pending.push_back(t);
+  s->st.submitted++;
+  *ticket = t;
+  s->cv_work.notify_one();
+  return ACC_OK;
+}
+
+int acc_server_wait(acc_server *s, uint64_t ticket, int32_t timeout_ms) {
+  if (!s) return acc_set_error(ACC_EINVAL, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto done = [&]() {
+    auto it = s->queries.find(ticket);
+    return it == s->queries.end() || it->second.state >= kDone;
+  };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lk, done);
+  else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+    return acc_set_error(ACC_ETIMEDOUT, "query not complete");
+  auto it = s->queries.find(ticket);
+  if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
+  if (it->second.state == kFailed) return acc_set_error(it->second.err, it->second.err_text);
+  return ACC_OK;
+}
+
+int acc_server_poll(acc_server *s, uint64_t ticket, int *state) {
+  if (!s || !state) return acc_set_error(ACC_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->queries.find(ticket);
+  if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
+  *state = it->second.state;
+  return ACC_OK;
+}
+
+int acc_server_result(acc_server *s, uint64_t ticket, const int32_t **label, const int32_t **cluster_id,
+                      uint32_t *n, int64_t *counts, double *latency_ms) {
+  if (!s) return acc_set_error(ACC_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->queries.find(ticket);
+  if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
+  const Query &q = it->second;
+  if (q.state == kFailed) return acc_set_error(q.err, q.err_text);
+  if (q.state != kDone) return acc_set_error(ACC_EINVAL, "query not complete");
+  const Slot &sl = s->slots[q.slot];
+  if (label) *label = sl.h_label;
+  if (cluster_id) *cluster_id = sl.h_cid;
+  if (n) *n = s->graphs[q.gid].n;
+  if (counts)
+    for (int i = 0; i < 4; i++) counts[i] = q.counts[i];
+  if (latency_ms) *latency_ms = ms_between(q.t_submit, q.t_done);
+  return ACC_OK;
+}
+
+int acc_server_release(acc_server *s, uint64_t ticket) {
+  if (!s) return acc_set_error(ACC_EINVAL, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->queries.find(ticket);
+  if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
+  Query &q = it->second;
+  if (q.state == kPending || q.state == kRunning) {
+    if (q.state == kRunning) return acc_set_error(ACC_EINVAL, "query is running");
+    // a pending query is cancelled
+    auto pi = std::find(s->pending.begin(), s->pending.end(), ticket);
+    if (pi != s->pending.end()) s->pending.erase(pi);
+    s->graphs[q.gid].waiting--;
+    s->st.cancelled++;
+  }
+  if (q.slot >= 0) s->slots[q.slot].busy = false;
+  s->queries.erase(it);
+  s->cv_work.notify_one();
+  return ACC_OK;
+}
+
+int acc_server_get_stats(acc_server *s, acc_server_stats *out) {
+  if (!s || !out) return acc_set_error(ACC_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *out = s->st;
+  out->graph_bytes = s->graph_bytes;
+  out->slot_bytes = s->slot_bytes;
+  out->resident_graphs = 0;
+  for (auto &kv : s->graphs) out->resident_graphs += kv.second.g != nullptr;
+  out->pending = s->pending.size();
+  return ACC_OK;
+}
+
+}  // extern "C"

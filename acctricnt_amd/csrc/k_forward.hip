@@ -10,10 +10,13 @@
 //   (c,a)  one global atomic into the lower slot (c,a), the probe element's
 //          own slot.
 // Upper slots therefore receive only plain stores and lower slots only
-// atomics.  Scanning N(c) up to b also yields the reverse slot of (b,c)
-// (the position of b in N(c)), which k_combine uses to add the two halves
-// and write 2 + count into both slots.  The counts equal the reference's
-// merge counts (scan-query/scan_xp.cpp:675-857) for every slot.
+// atomics.  The reverse slot of every upper slot (b,c), the position of b in
+// N(c), is computed once per resident graph (k_build_rev); it bounds the
+// probe of N(c) to exactly |N(c) ∩ [0, b)| elements, which lets a wave or a
+// workgroup flatten all probes of a source into one element range with no
+// idle lane, and k_combine uses it to add the two halves and write
+// 2 + count into both slots.  The counts equal the reference's merge counts
+// (scan-query/scan_xp.cpp:675-857) for every slot.
 
 #include "acc_internal.h"
 
@@ -42,22 +45,47 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint32_t *__restrict__ rev,
-                                              uint64_t m, unsigned long long *__restrict__ sum) {
+__global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_t m,
+                                              unsigned long long *__restrict__ sum) {
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
   const uint64_t stride = gridDim.x * 256ull;
   const uint64_t m4 = m / 4;
   const int4 z = make_int4(0, 0, 0, 0);
-  const uint4 f = make_uint4(~0u, ~0u, ~0u, ~0u);
-  for (uint64_t i = tid; i < m4; i += stride) {
-    reinterpret_cast<int4 *>(cnt)[i] = z;
-    reinterpret_cast<uint4 *>(rev)[i] = f;
-  }
-  for (uint64_t i = m4 * 4 + tid; i < m; i += stride) {
-    cnt[i] = 0;
-    rev[i] = ~0u;
-  }
+  for (uint64_t i = tid; i < m4; i += stride) reinterpret_cast<int4 *>(cnt)[i] = z;
+  for (uint64_t i = m4 * 4 + tid; i < m; i += stride) cnt[i] = 0;
   if (tid == 0) *sum = 0;
+}
+
+// Reverse slots, computed once per resident graph: rev[j] = position of u in
+// N(v) for every upper slot j = (u, v), u < v (binary search of N(v)), ~0
+// for lower slots and for an upper slot whose reverse is absent (possible
+// only in an unvalidated, asymmetric CSR).  One wave per vertex.
+__global__ __launch_bounds__(256) void k_build_rev(const uint32_t *__restrict__ off,
+                                                   const int32_t *__restrict__ adj, uint32_t n,
+                                                   uint32_t *__restrict__ rev) {
+  const uint32_t waves = gridDim.x * 4u;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t v = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6)); v < n; v += waves) {
+    const uint32_t s = off[v], e = off[v + 1];
+    for (uint32_t j = s + lane; j < e; j += 64) {
+      const int32_t c = adj[j];
+      uint32_t p = ~0u;
+      if (c > static_cast<int32_t>(v)) {
+        uint32_t lo = off[c];
+        const uint32_t end = off[c + 1];
+        uint32_t hi = end;
+        while (lo < hi) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (adj[mid] < static_cast<int32_t>(v))
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (lo < end && adj[lo] == static_cast<int32_t>(v)) p = lo;
+      }
+      rev[j] = p;
+    }
+  }
 }
 
 // Makes the wave's earlier LDS writes visible to all of its lanes before
@@ -125,104 +153,154 @@ struct BitSet {
   }
 };
 
-// Scans N(c) from its start up to b, U 64-element chunks per round (the loads
-// of a round are independent, so U sets the memory-level parallelism).  Every
-// element a < b found in N<(b) is a triangle {a, b, c}: count it for (b,c) in
-// the returned total, for (b,a) through ctr[pos] (LDS, positions below
-// ctr_cap) or an atomic on the lower slot sb+pos, and for (c,a) through an
-// atomic on the element's own slot.  *p receives the position of b in N(c).
+// Exclusive prefix sum of v over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
+// Records one triangle {a, b, c} found at element slot idx of N(c): the
+// (c,a) slot itself and the (b,a) counter of a's position pos in N<(b)
+// (LDS below ctr_cap, else an atomic on the lower slot sb+pos).
+__device__ __forceinline__ void count_triangle(int32_t *cnt, int32_t *ctr, uint32_t ctr_cap, uint32_t sb,
+                                               uint32_t idx, int pos) {
+  atomicAdd(&cnt[idx], 1);
+  if (static_cast<uint32_t>(pos) < ctr_cap)
+    atomicAdd(&ctr[pos], 1);
+  else
+    atomicAdd(&cnt[sb + pos], 1);
+}
+
+// Adds the round's hits to the (b,c_k) counters reg[k] (LDS): one add of the
+// ballot's popcount when the whole round lies in one probe, else one LDS
+// atomic per hit.
+__device__ __forceinline__ void count_round(int32_t *reg, int k, bool hit, int lane) {
+  const unsigned long long hits = __ballot(hit);
+  if (!hits) return;
+  const int k0 = __builtin_amdgcn_readfirstlane(k);
+  if (k0 == __builtin_amdgcn_readlane(k, 63)) {
+    if (lane == 0) atomicAdd(&reg[k0], static_cast<int>(__popcll(hits)));
+  } else if (hit) {
+    atomicAdd(&reg[k], 1);
+  }
+}
+
+// Wave-flattened probing of up to 64 upper slots (b, c_k), one per lane k:
+// lane k holds cb = start of N(c_k) and P = the exclusive prefix of the
+// probe lengths L_k = |N(c_k) ∩ [0, b)| (lanes without a probe hold P = T,
+// the total).  Element e of the flattened range [0, T) belongs to probe
+// k(e) = max{k : P_k <= e} (a 6-step search over the lanes) and is
+// N(c_k)[e - P_k]; every lane takes one element per round, U rounds per
+// iteration, so short lists leave no lane idle and no load passes b.
 template <int U, class Set>
-__device__ __forceinline__ int scan_probe(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
-                                          int32_t *ctr, uint32_t ctr_cap, uint32_t sb, int32_t b, uint32_t cb,
-                                          uint32_t ce, int lane, uint32_t *p) {
-  int reg = 0;
-  for (uint32_t base = cb;; base += 64 * U) {
+__device__ __forceinline__ void wave_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
+                                                int32_t *ctr, uint32_t ctr_cap, uint32_t sb, uint32_t P,
+                                                uint32_t cb, uint32_t T, int32_t *reg, int lane) {
+  for (uint32_t e0 = 0; e0 < T; e0 += 64 * U) {
     int32_t a[U];
+    uint32_t idx[U];
+    int kq[U];
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      const uint32_t i = base + 64 * q + lane;
-      a[q] = i < ce ? adj[i] : INT32_MAX;
+      const uint32_t e = e0 + 64 * q + lane;
+      int k = 0;
+#pragma unroll
+      for (int st = 32; st; st >>= 1)
+        if (static_cast<uint32_t>(__shfl(P, k + st)) <= e) k += st;
+      kq[q] = k;
+      idx[q] = static_cast<uint32_t>(__shfl(cb, k)) + (e - static_cast<uint32_t>(__shfl(P, k)));
+      a[q] = e < T ? adj[idx[q]] : -1;
     }
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      if (a[q] < b) {
+      bool hit = false;
+      if (a[q] >= 0) {
         const int pos = set.find(a[q]);
         if (pos >= 0) {
-          reg++;
-          atomicAdd(&cnt[base + 64 * q + lane], 1);
-          if (static_cast<uint32_t>(pos) < ctr_cap)
-            atomicAdd(&ctr[pos], 1);
-          else
-            atomicAdd(&cnt[sb + pos], 1);
+          count_triangle(cnt, ctr, ctr_cap, sb, idx[q], pos);
+          hit = true;
         }
       }
+      count_round(reg, kq[q], hit, lane);
     }
-    bool done = false;
-#pragma unroll
-    for (int q = 0; q < U; q++) {
-      const unsigned long long m = __ballot(a[q] < b);
-      if (m != ~0ull) {
-        *p = base + 64 * q + static_cast<uint32_t>(__builtin_ctzll(~m));
-        done = true;
-        break;
-      }
-    }
-    if (done) break;
   }
-  if (*p >= ce) *p = ~0u;  // b absent from N(c): only possible for an unvalidated, asymmetric CSR
-  return reg;
 }
 
 constexpr int kWT = 1 << accfwd::kWaveTableBits;
 constexpr int kBT = 1 << kTableBits;
-constexpr size_t kBlockLdsBytes = 4 * kBT + 2 * kBT + 4 * kLdsCap + 8 * kChunk;
-constexpr size_t kWaveLdsBytes = kWaves * (4 * kWT + 2 * kWT + 4 * accfwd::kWaveCap);
+constexpr size_t kBlockLdsBytes = 4 * kBT + 2 * kBT + 4 * kLdsCap + 12 * kChunk + 4 * kWaves;
+constexpr size_t kWaveLdsBytes = kWaves * (4 * kWT + 2 * kWT + 4 * accfwd::kWaveCap + 4 * 64);
 constexpr size_t kCountLdsBytes = kBlockLdsBytes > kWaveLdsBytes ? kBlockLdsBytes : kWaveLdsBytes;
 
-// Runs the neighbor loop of a workgroup unit once the set of N<(b) is built.
+// Workgroup-flattened probing of a chunk's nup <= kChunk upper slots: s_P
+// (kChunk entries, T beyond nup) holds the exclusive prefix of the probe
+// lengths and s_cb the starts of the N(c_k).  Wave w takes the rounds
+// w, w + kWaves, ... of 64 * U consecutive elements, so the chunk's work is
+// split evenly over the waves whatever the list lengths; k(e) is a 9-step
+// search of s_P.  Round indices are wave-uniform.
 template <int U, class Set>
-__device__ __forceinline__ void fwd_block_loop(const int32_t *__restrict__ adj, const FwdUnit u, int32_t *cnt,
-                                               uint32_t *__restrict__ rev, const Set &set, int32_t *ctr,
-                                               const uint32_t *s_cb, const uint32_t *s_ce, uint32_t sb,
-                                               uint32_t nup) {
-  // Static round-robin of neighbors over waves.  The loop index must stay
-  // wave-uniform and must not be produced by a lane-0-only branch: a work
-  // counter taken by lane 0 and broadcast lets the compiler rotate the loop so
-  // that lanes 1..63 re-enter it without lane 0 and never exit.
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  for (uint32_t k = wave; k < nup; k += kWaves) {
-    uint32_t p = ~0u;
-    const uint32_t cb = __builtin_amdgcn_readfirstlane(s_cb[k]);
-    const uint32_t ce = __builtin_amdgcn_readfirstlane(s_ce[k]);
-    int reg = scan_probe<U>(adj, cnt, set, ctr, kLdsCap, sb, u.b, cb, ce, lane, &p);
-    reg = wave_sum_i32(reg);
-    if (lane == 0) {
-      cnt[u.j0 + k] = reg;
-      rev[u.j0 + k] = p;
+__device__ __forceinline__ void block_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
+                                                 int32_t *ctr, uint32_t sb, const uint32_t *s_P,
+                                                 const uint32_t *s_cb, uint32_t T, int32_t *s_reg, int wave,
+                                                 int lane) {
+  static_assert(kChunk == 512, "the search below covers 512 entries");
+  for (uint32_t e0 = static_cast<uint32_t>(wave) * 64 * U; e0 < T; e0 += kWaves * 64 * U) {
+    int32_t a[U];
+    uint32_t idx[U];
+    int kq[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t e = e0 + 64 * q + lane;
+      int k = 0;
+#pragma unroll
+      for (int st = 256; st; st >>= 1)
+        if (s_P[k + st] <= e) k += st;
+      kq[q] = k;
+      idx[q] = s_cb[k] + (e - s_P[k]);
+      a[q] = e < T ? adj[idx[q]] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      bool hit = false;
+      if (a[q] >= 0) {
+        const int pos = set.find(a[q]);
+        if (pos >= 0) {
+          count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
+          hit = true;
+        }
+      }
+      count_round(s_reg, kq[q], hit, lane);
     }
   }
 }
 
 // Workgroup unit: one chunk of at most kChunk upper slots of a source b.
 // N<(b) is an LDS bitmap when b - min N(b) <= kBitmapIds (|N<(b)| < 65536),
-// otherwise an
-// LDS hash table (|N<(b)| <= kLdsCap).  Both layouts share the ctr and
-// neighbor-metadata regions of smem.
+// otherwise an LDS hash table (|N<(b)| <= kLdsCap).  Both layouts share the
+// counter and probe-metadata regions of smem.
 template <int U>
 __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                               const FwdUnit u, int32_t *cnt, uint32_t *__restrict__ rev,
+                                               const uint32_t *__restrict__ rev, const FwdUnit u, int32_t *cnt,
                                                unsigned char *smem) {
   int32_t *keys = reinterpret_cast<int32_t *>(smem);  // hash keys, or bitmap words
   int32_t *ctr = keys + kBT;
   uint32_t *s_cb = reinterpret_cast<uint32_t *>(ctr + kLdsCap);
-  uint32_t *s_ce = s_cb + kChunk;
-  uint16_t *kpos = reinterpret_cast<uint16_t *>(s_ce + kChunk);  // hash positions, or word prefixes
+  uint32_t *s_P = s_cb + kChunk;
+  int32_t *s_reg = reinterpret_cast<int32_t *>(s_P + kChunk);
+  uint32_t *s_wsum = reinterpret_cast<uint32_t *>(s_reg + kChunk);
+  uint16_t *kpos = reinterpret_cast<uint16_t *>(s_wsum + kWaves);  // hash positions, or word prefixes
   const int32_t b = u.b;
   const uint32_t sb = off[b];
   const uint32_t nl = u.nlow;
   const uint32_t nup = u.j1 - u.j0;
   const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
   const int32_t lo = nl ? adj[sb] : b;
   const uint32_t span = static_cast<uint32_t>(b - lo);
   const bool bitmap = span <= accfwd::kBitmapIds;
@@ -233,12 +311,27 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   const int32_t fill = bitmap ? 0 : -1;
   for (uint32_t i = tid; i < nclear; i += kWG) keys[i] = fill;
   for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
-  for (uint32_t i = tid; i < nup; i += kWG) {
-    const int32_t c = adj[u.j0 + i];
-    s_cb[i] = off[c];
-    s_ce[i] = off[c + 1];
+  // probe lengths L_t = |N(c_t) ∩ [0, b)| = rev - start of N(c_t), and their prefix
+  uint32_t len = 0;
+  if (static_cast<uint32_t>(tid) < nup) {
+    const int32_t c = adj[u.j0 + tid];
+    const uint32_t cb = off[c];
+    const uint32_t p = rev[u.j0 + tid];
+    len = p != ~0u ? p - cb : 0;
+    s_cb[tid] = cb;
   }
+  s_reg[tid] = 0;
+  const uint32_t excl = wave_excl_scan(len, lane);
+  if (lane == 63) s_wsum[wave] = excl + len;
   __syncthreads();
+  uint32_t before = 0, T = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    const uint32_t ws = s_wsum[w];
+    before += w < wave ? ws : 0;
+    T += ws;
+  }
+  s_P[tid] = static_cast<uint32_t>(tid) < nup ? before + excl : T;
   if (bitmap) {
     uint32_t *bm = reinterpret_cast<uint32_t *>(keys);
     for (uint32_t i = tid; i < nl; i += kWG) {
@@ -250,14 +343,16 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
     lds_insert(keys, kpos, bits, mask, adj, sb, nl, tid, kWG);
   }
   __syncthreads();
+  T = __builtin_amdgcn_readfirstlane(T);
   if (bitmap) {
     const BitSet set{reinterpret_cast<const uint32_t *>(keys), kpos, lo};
-    fwd_block_loop<U>(adj, u, cnt, rev, set, ctr, s_cb, s_ce, sb, nup);
+    block_probe_flat<U>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
   } else {
     const HashSet set{keys, kpos, bits, mask};
-    fwd_block_loop<U>(adj, u, cnt, rev, set, ctr, s_cb, s_ce, sb, nup);
+    block_probe_flat<U>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
   }
   __syncthreads();
+  if (static_cast<uint32_t>(tid) < nup) cnt[u.j0 + tid] = s_reg[tid];
   for (uint32_t i = tid; i < nctr; i += kWG) {
     const int v = ctr[i];
     if (v) atomicAdd(&cnt[sb + i], v);
@@ -268,12 +363,13 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
 // |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.
 template <int U>
 __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                              const FwdUnit u, int32_t *cnt, uint32_t *__restrict__ rev,
+                                              const uint32_t *__restrict__ rev, const FwdUnit u, int32_t *cnt,
                                               unsigned char *smem, int wave) {
-  int32_t *keys = reinterpret_cast<int32_t *>(smem) + wave * kWT;
-  int32_t *ctr = reinterpret_cast<int32_t *>(smem) + kWaves * kWT + wave * accfwd::kWaveCap;
-  uint16_t *kpos = reinterpret_cast<uint16_t *>(reinterpret_cast<int32_t *>(smem) + kWaves * kWT +
-                                                kWaves * accfwd::kWaveCap) + wave * kWT;
+  int32_t *base = reinterpret_cast<int32_t *>(smem);
+  int32_t *keys = base + wave * kWT;
+  int32_t *ctr = base + kWaves * kWT + wave * accfwd::kWaveCap;
+  int32_t *reg = base + kWaves * (kWT + accfwd::kWaveCap) + wave * 64;
+  uint16_t *kpos = reinterpret_cast<uint16_t *>(base + kWaves * (kWT + accfwd::kWaveCap + 64)) + wave * kWT;
   const int lane = threadIdx.x & 63;
   const int32_t b = u.b;
   const uint32_t sb = off[b];
@@ -281,28 +377,26 @@ __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, 
   const uint32_t nup = u.j1 - u.j0;
   const int bits = table_bits_for(nl);
   const uint32_t mask = (1u << bits) - 1;
-  // neighbor metadata, one upper slot per lane
-  const int32_t my_c = lane < static_cast<int>(nup) ? adj[u.j0 + lane] : 0;
-  const uint32_t my_cb = lane < static_cast<int>(nup) ? off[my_c] : 0;
-  const uint32_t my_ce = lane < static_cast<int>(nup) ? off[my_c + 1] : 0;
+  // one upper slot per lane: start of N(c) and probe length |N(c) ∩ [0, b)|
+  uint32_t cb = 0, len = 0;
+  if (static_cast<uint32_t>(lane) < nup) {
+    const int32_t c = adj[u.j0 + lane];
+    cb = off[c];
+    const uint32_t p = rev[u.j0 + lane];
+    len = p != ~0u ? p - cb : 0;
+  }
+  const uint32_t P = wave_excl_scan(len, lane);
+  const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
   for (uint32_t i = lane; i <= mask; i += 64) keys[i] = -1;
   for (uint32_t i = lane; i < nl; i += 64) ctr[i] = 0;
+  reg[lane] = 0;
   wave_lds_sync();
   lds_insert(keys, kpos, bits, mask, adj, sb, nl, lane, 64);
   wave_lds_sync();
-  for (uint32_t k = 0; k < nup; k++) {
-    const uint32_t cb = __builtin_amdgcn_readfirstlane(__shfl(my_cb, k));
-    const uint32_t ce = __builtin_amdgcn_readfirstlane(__shfl(my_ce, k));
-    uint32_t p = ~0u;
-    const HashSet set{keys, kpos, bits, mask};
-    int reg = scan_probe<U>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, b, cb, ce, lane, &p);
-    reg = wave_sum_i32(reg);
-    if (lane == 0) {
-      cnt[u.j0 + k] = reg;
-      rev[u.j0 + k] = p;
-    }
-  }
+  const HashSet set{keys, kpos, bits, mask};
+  wave_probe_flat<U>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, P, cb, T, reg, lane);
   wave_lds_sync();
+  if (static_cast<uint32_t>(lane) < nup) cnt[u.j0 + lane] = reg[lane];
   for (uint32_t i = lane; i < nl; i += 64) {
     const int v = ctr[i];
     if (v) atomicAdd(&cnt[sb + i], v);
@@ -317,29 +411,31 @@ __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, 
 template <int U>
 __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
                                                    const int32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ rev,
                                                    const FwdUnit *__restrict__ block_units, uint32_t n_block,
                                                    const FwdUnit *__restrict__ wave_units, uint32_t n_wave,
-                                                   int32_t *cnt, uint32_t *__restrict__ rev) {
+                                                   int32_t *cnt) {
   __shared__ __align__(16) unsigned char smem[kCountLdsBytes];
   const uint32_t tasks = n_block + (n_wave + kWaves - 1) / kWaves;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t t = blockIdx.x; t < tasks; t += gridDim.x) {
     if (t < n_block) {
-      fwd_block_unit<U>(off, adj, block_units[t], cnt, rev, smem);
+      fwd_block_unit<U>(off, adj, rev, block_units[t], cnt, smem);
     } else {
       const uint32_t ui = (t - n_block) * kWaves + wave;
-      if (ui < n_wave) fwd_wave_unit<U>(off, adj, wave_units[ui], cnt, rev, smem, wave);
+      if (ui < n_wave) fwd_wave_unit<U>(off, adj, rev, wave_units[ui], cnt, smem, wave);
     }
     __syncthreads();
   }
 }
 
 // Sources whose N<(b) exceeds the workgroup table: membership by binary
-// search of N<(b) in global memory, (b,a) counted by direct atomics.
+// search of N<(b) in global memory, (b,a) counted by direct atomics.  One
+// wave per upper slot; the probe stops at the reverse slot.
 __global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__restrict__ off,
                                                             const int32_t *__restrict__ adj,
-                                                            const FwdUnit *__restrict__ units, int32_t *cnt,
-                                                            uint32_t *__restrict__ rev) {
+                                                            const uint32_t *__restrict__ rev,
+                                                            const FwdUnit *__restrict__ units, int32_t *cnt) {
   const FwdUnit u = units[blockIdx.x];
   const int32_t b = u.b;
   const uint32_t sb = off[b];
@@ -349,14 +445,13 @@ __global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__re
   for (uint32_t j = u.j0 + wave; j < u.j1; j += accfwd::kBigWG / 64) {
     const int32_t c = adj[j];
     const uint32_t cb = off[c];
-    const uint32_t ce = off[c + 1];
+    const uint32_t p = rev[j];
+    const uint32_t ce = p != ~0u ? p : cb;
     int reg = 0;
-    uint32_t p = ~0u;
-    for (uint32_t base = cb;; base += 64) {
+    for (uint32_t base = cb; base < ce; base += 64) {
       const uint32_t idx = base + lane;
-      const int32_t a = idx < ce ? adj[idx] : INT32_MAX;
-      const bool lt = a < b;
-      if (lt) {
+      if (idx < ce) {
+        const int32_t a = adj[idx];
         uint32_t lo = 0, hi = nl;
         while (lo < hi) {
           const uint32_t mid = (lo + hi) >> 1;
@@ -371,18 +466,9 @@ __global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__re
           atomicAdd(&cnt[sb + lo], 1);
         }
       }
-      const unsigned long long mlt = __ballot(lt);
-      if (mlt != ~0ull) {
-        p = base + static_cast<uint32_t>(__builtin_ctzll(~mlt));
-        if (p >= ce) p = ~0u;
-        break;
-      }
     }
     reg = wave_sum_i32(reg);
-    if (lane == 0) {
-      cnt[j] = reg;
-      rev[j] = p;
-    }
+    if (lane == 0) cnt[j] = reg;
   }
 }
 
@@ -500,7 +586,17 @@ namespace accfwd {
 
 int launch_init(acc_graph *g) {
   hipLaunchKernelGGL(k_init, dim3(grid_for(g->m / 4 + 1, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream,
-                     g->d_cnt, g->d_rev, g->m, g->d_sum);
+                     g->d_cnt, g->m, g->d_sum);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_build_rev(acc_graph *g) {
+  if (!g->n) return ACC_OK;
+  const uint64_t want = (static_cast<uint64_t>(g->n) + 3) / 4;
+  const uint64_t cap = static_cast<uint64_t>(g->ctx->num_cus > 0 ? g->ctx->num_cus : 256) * 64;
+  hipLaunchKernelGGL(k_build_rev, dim3(static_cast<unsigned>(want < cap ? want : cap)), dim3(256), 0,
+                     g->ctx->stream, g->d_off, g->d_adj, g->n, g->d_rev);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
@@ -513,16 +609,16 @@ int launch_count(acc_graph *g) {
   if (grid) {
     switch (g->fwd_unroll) {
       case 1:
-        hipLaunchKernelGGL(k_fwd_count<1>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
-                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        hipLaunchKernelGGL(k_fwd_count<1>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
         break;
       case 4:
-        hipLaunchKernelGGL(k_fwd_count<4>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
-                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        hipLaunchKernelGGL(k_fwd_count<4>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
         break;
       default:
-        hipLaunchKernelGGL(k_fwd_count<2>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_fwd_block,
-                           g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, g->d_rev);
+        hipLaunchKernelGGL(k_fwd_count<2>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
         break;
     }
     ACC_HIP_TRY(hipGetLastError());
@@ -533,7 +629,7 @@ int launch_count(acc_graph *g) {
 int launch_count_big(acc_graph *g) {
   if (g->n_fwd_big) {
     hipLaunchKernelGGL(k_fwd_big, dim3(g->n_fwd_big), dim3(kBigWG), 0, g->ctx->stream, g->d_off, g->d_adj,
-                       g->d_fwd_big, g->d_cnt, g->d_rev);
+                       g->d_rev, g->d_fwd_big, g->d_cnt);
     ACC_HIP_TRY(hipGetLastError());
   }
   return ACC_OK;

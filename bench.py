@@ -32,6 +32,7 @@ WORKLOADS = {
     "rmat20": dict(kind="rmat", scale=20, edge_factor=16, seed=1),
     "chunglu23": dict(kind="chung_lu", n=1 << 23, gamma=2.1, avg_deg=32, max_deg=1 << 20, seed=1),
     "er16384": dict(kind="er", n=16384, avg_deg=16, seed=20190101),
+    "batch512": dict(kind="batch", graphs=512),
 }
 
 
@@ -40,6 +41,10 @@ def make_graph(spec):
 
     if spec["kind"] == "rmat":
         return G.gen_rmat(spec["scale"], spec["edge_factor"], seed=spec["seed"])
+    if spec["kind"] == "batch":
+        packed, vo = G.pack(G.batch_mix(num_graphs=spec["graphs"]))
+        packed.meta["vertex_off"] = vo
+        return packed
     if spec["kind"] == "chung_lu":
         return G.gen_chung_lu(spec["n"], spec["gamma"], spec["avg_deg"], spec["max_deg"], seed=spec["seed"])
     return G.gen_er(spec["n"], spec["avg_deg"], seed=spec["seed"])
@@ -185,6 +190,95 @@ def bench_batch(ctx, steps: int, warmup: int, num_graphs: int = 512):
     }
 
 
+def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16, seed: int = 5):
+    """Serving form: `graphs` resident R-MAT graphs (scale uniform in 16..22,
+    edge factor 16, degree-relabelled), SCAN queries (graph uniform, eps
+    uniform in {0.1, ..., 0.9}, mu uniform in {2, 5, 10, 20}) arriving as a
+    Poisson process at `rate` per second.  Latency runs from submission to
+    the labels being in host memory; a sample of results is compared with
+    the same query run alone."""
+    import threading
+
+    import numpy as np
+
+    from acctricnt_amd import graph as G, serve
+
+    rng = np.random.default_rng(seed)
+    scales = [int(x) for x in rng.integers(16, 23, size=graphs)]
+    t0 = time.perf_counter()
+    gs = [G.gen_rmat(sc, 16, seed=1000 + i) for i, sc in enumerate(scales)]
+    print(f"[serve] {graphs} R-MAT graphs, scales {scales}, {sum(g.m for g in gs)} slots, "
+          f"gen {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    sv = serve.Server(ctx, max_batch=16, streams=4, slots=64)
+    t0 = time.perf_counter()
+    gids = [sv.add_graph(g) for g in gs]
+    t_admit = time.perf_counter() - t0
+    epss = [f"0.{k}" for k in range(1, 10)]
+    mus = [2, 5, 10, 20]
+    # warm-up: grow every result slot to the largest graph
+    big = int(np.argmax([g.m for g in gs]))
+    warm = [sv.submit(gids[big], "0.5", 5) for _ in range(64)]
+    for t in warm:
+        sv.wait(t)
+        sv.release(t)
+    plan = [(int(rng.integers(graphs)), epss[int(rng.integers(9))], mus[int(rng.integers(4))])
+            for _ in range(queries)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=queries))
+    keep = set(int(i) for i in rng.choice(queries, size=min(48, queries), replace=False))
+    tickets = [None] * queries
+    lat = np.zeros(queries)
+    kept = {}
+    ready = threading.Semaphore(0)
+
+    def collect():
+        for i in range(queries):
+            ready.acquire()
+            t = tickets[i]
+            sv.wait(t)
+            r = sv.result(t, copy=i in keep)
+            lat[i] = r["latency_ms"]
+            if i in keep:
+                kept[i] = r
+            sv.release(t)
+
+    before = sv.stats()
+    th = threading.Thread(target=collect)
+    th.start()
+    start = time.perf_counter()
+    for i, (k, eps, mu) in enumerate(plan):
+        while time.perf_counter() - start < arrivals[i]:
+            pass
+        tickets[i] = sv.submit(gids[k], eps, mu)
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - start
+    after = sv.stats()
+    same = True
+    for i, r in kept.items():
+        k, eps, mu = plan[i]
+        with ctx.upload(gs[k]) as rg, rg.scan_workspace() as ws:
+            ws.run(eps, mu)
+            ws.synchronize()
+            a = ws.results()
+        same = same and bool(np.array_equal(a["label"], r["label"]) and np.array_equal(a["cluster_id"],
+                                                                                         r["cluster_id"]))
+    batches = after["batches"] - before["batches"]
+    out = {
+        "graphs": graphs, "scales": scales, "queries": queries, "arrival_rate_qps": rate,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(lat.max()), "mean_ms": float(lat.mean()),
+        "queries_per_s": queries / wall, "batches": batches,
+        "mean_batch": (after["batched_queries"] - before["batched_queries"]) / max(batches, 1),
+        "failed": after["failed"] - before["failed"], "admit_s": t_admit,
+        "resident_gb": after["graph_bytes"] / 1e9, "slot_gb": after["slot_bytes"] / 1e9,
+        "identical_alone_sampled": same, "sampled": len(kept),
+    }
+    sv.close()
+    print(f"[serve] p50 {out['p50_ms']:.3f} ms p99 {out['p99_ms']:.3f} ms, {out['queries_per_s']:.0f} q/s, "
+          f"mean batch {out['mean_batch']:.2f}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +292,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain measurement")
     ap.add_argument("--batch-graphs", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
+    ap.add_argument("--serve-queries", type=int, default=6000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -223,6 +320,8 @@ def main():
         calibration_copy(args.calibrate_copy)
     t0 = time.perf_counter()
     rg = ctx.upload(g)
+    if "vertex_off" in g.meta:
+        rg.set_partition(g.meta["vertex_off"])
     t_up = time.perf_counter() - t0
     params = {}
     for kv in args.param:
@@ -253,6 +352,7 @@ def main():
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
     rg.release()
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
+    served = None if args.no_serve else bench_serve(ctx, args.serve_queries, args.serve_rate)
     ctx.close()
 
     cpu = None if args.no_cpu else cpu_baseline(g, args.cpu_seconds)
@@ -281,6 +381,7 @@ def main():
                      "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
         "batch": batch,
+        "serve": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -163,6 +163,65 @@ int acc_scan_results(acc_scan_ws *ws, int32_t *label, int32_t *cluster_id, uint8
  * unordered; *count receives the total (which may exceed cap). */
 int acc_scan_pairs(acc_scan_ws *ws, int32_t *pairs, uint64_t cap, uint64_t *count);
 
+/* Serving ------------------------------------------------------------------
+ * A server keeps graphs resident in HBM with their counts computed once at
+ * admission, admits and evicts them by device footprint against a budget
+ * (least recently used idle graphs go first; an evicted graph is admitted
+ * again from the server's host copy when a query needs it), and answers
+ * SCAN queries (graph, eps = eps_num/eps_den, mu) submitted from any thread.
+ * A worker thread batches pending queries across graphs into one set of SCAN
+ * launches per batch, on one of `streams` streams, while earlier batches are
+ * still running (continuous batching).  Each query's labels (0 unclassified,
+ * 1 core, 2 hub, as acc_scan_results) and cluster ids (-1 for non-cores) are
+ * the ones acc_scan_run gives for the same graph and parameters alone; they
+ * are copied to pinned host memory, readable with acc_server_result until
+ * acc_server_release.  No counterpart in the reference (its SCAN_XP runs one
+ * query per process, scan-query/scan_xp.cpp:2083-2110). */
+typedef struct acc_server acc_server;
+typedef struct acc_server_config {
+  uint64_t hbm_budget; /* device bytes for resident graphs plus result slots */
+  int32_t max_batch;   /* queries per batched launch, 1..256 */
+  int32_t streams;     /* batches in flight at once, 1..16 */
+  int32_t slots;       /* queries holding results at once (launched, not released), 1..4096 */
+  int32_t reserved;
+} acc_server_config;
+typedef struct acc_server_stats {
+  uint64_t submitted, completed, failed, cancelled;
+  uint64_t batches, batched_queries, max_batch_seen;
+  uint64_t admissions, evictions;
+  uint64_t graph_bytes, slot_bytes, resident_graphs, pending;
+} acc_server_stats;
+
+#define ACC_QUERY_PENDING 0
+#define ACC_QUERY_RUNNING 1
+#define ACC_QUERY_DONE 2
+#define ACC_QUERY_FAILED 3
+
+void acc_server_config_default(acc_server_config *cfg); /* 64 GiB, batch 16, 4 streams, 32 slots */
+int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **out);
+void acc_server_destroy(acc_server *s);
+/* Registers a CSR (copied) and admits it; the first admission validates it
+ * unless flags has ACC_UPLOAD_NO_VALIDATE.  ACC_EBUDGET if it cannot fit. */
+int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
+                         uint32_t flags, int32_t *graph_id);
+int acc_server_remove_graph(acc_server *s, int32_t graph_id);
+int acc_server_graph_info(acc_server *s, int32_t graph_id, uint32_t *n, uint64_t *m, int *resident,
+                          uint64_t *device_bytes);
+int acc_server_submit(acc_server *s, int32_t graph_id, int64_t eps_num, int64_t eps_den, int32_t mu,
+                      uint64_t *ticket);
+/* Waits for a query: ACC_OK when done, ACC_ETIMEDOUT after timeout_ms (< 0: no limit),
+ * the query's own error code when it failed. */
+int acc_server_wait(acc_server *s, uint64_t ticket, int32_t timeout_ms);
+int acc_server_poll(acc_server *s, uint64_t ticket, int *state);
+/* Results of a completed query: pointers into pinned memory valid until
+ * acc_server_release; counts = cores, clusters, hubs, outliers; latency from
+ * submission to completion. */
+int acc_server_result(acc_server *s, uint64_t ticket, const int32_t **label, const int32_t **cluster_id,
+                      uint32_t *n, int64_t *counts, double *latency_ms);
+/* Frees a query's result slot; a pending query is cancelled. */
+int acc_server_release(acc_server *s, uint64_t ticket);
+int acc_server_get_stats(acc_server *s, acc_server_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

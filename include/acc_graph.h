@@ -32,7 +32,8 @@ typedef struct acc_hgraph acc_hgraph; /* opaque host CSR */
 #define ACC_EIO (-4)      /* file could not be opened, read or written */
 #define ACC_EDEVICE (-5)  /* HIP runtime error, kernel launch failure or missing device */
 #define ACC_EBUDGET (-6)  /* request exceeds a configured memory budget */
-#define ACC_ENOTFOUND (-7)
+#define ACC_ENOTFOUND (-7) /* unknown graph id, ticket or file entry */
+#define ACC_ETIMEDOUT (-8) /* a wait ended before the work completed */
 
 const char *acc_graph_last_error(void);
 

@@ -1,0 +1,175 @@
+"""SCAN serving over resident graphs (acc_server_*), through the C ABI.
+
+Every served query must return exactly the labels, cluster ids and counters
+of the oracle's SCAN for that graph and (eps, mu), and of the same query run
+alone through a SCAN workspace, whatever the other queries in its batch and
+whatever was admitted or evicted meanwhile.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle
+from acctricnt_amd import _ffi, cn, graph as G, serve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(built):
+    assert cn.device_count() > 0, "no HIP device visible"
+    c = cn.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    gs = [G.gen_rmat(12, 16, seed=61), G.gen_er(3000, 12, seed=62), G.gen_chung_lu(5000, 2.1, 16, 800, seed=63),
+          G.gen_rmat(14, 16, seed=64), G.from_edges(5, [0, 0, 0, 1, 1, 2, 3], [1, 2, 3, 2, 3, 3, 4])]
+    cnts = [oracle.cn_merge(g) for g in gs]
+    return gs, cnts
+
+
+QUERIES = [(e, mu) for e in ("0.1", "0.3", "0.5", "0.7", "0.9") for mu in (2, 5, 10)]
+
+
+def check(res, g, cnt, eps, mu):
+    ref = oracle.scan(g, cnt, eps, mu)
+    assert np.array_equal(res["label"], ref["label"]), f"labels differ eps={eps} mu={mu}"
+    core = ref["label"] == 1
+    assert np.array_equal(res["cluster_id"][core], ref["cluster_id"][core])
+    assert (res["cluster_id"][~core] == -1).all()
+    for k in ("core_num", "cluster_num", "hub_num", "outlier_num"):
+        assert res[k] == ref[k], k
+
+
+def test_serve_matches_oracle_and_alone(ctx, graphs):
+    gs, cnts = graphs
+    rng = np.random.default_rng(7)
+    with serve.Server(ctx, max_batch=8, streams=3, slots=48) as sv:
+        gids = [sv.add_graph(g) for g in gs]
+        # the mix: all queries submitted at once, batched across graphs
+        mix = [(int(rng.integers(len(gs))), *QUERIES[int(rng.integers(len(QUERIES)))]) for _ in range(120)]
+        tickets = [sv.submit(gids[k], eps, mu) for k, eps, mu in mix]
+        results = []
+        for t in tickets:
+            sv.wait(t)
+            results.append(sv.result(t))
+            sv.release(t)
+        st = sv.stats()
+        print(f"[serve] {st}", flush=True)
+        assert st["completed"] == len(mix) and st["failed"] == 0
+        assert st["max_batch_seen"] > 1, "queries were never batched"
+        # the same queries alone, through a workspace
+        alone = {}
+        for k, eps, mu in set(mix):
+            with ctx.upload(gs[k]) as rg, rg.scan_workspace() as ws:
+                ws.run(eps, mu)
+                ws.synchronize()
+                alone[(k, eps, mu)] = ws.results()
+        for (k, eps, mu), res in zip(mix, results):
+            a = alone[(k, eps, mu)]
+            assert np.array_equal(res["label"], a["label"]) and np.array_equal(res["cluster_id"], a["cluster_id"])
+        for k, eps, mu in sorted(set(mix))[:40]:
+            i = mix.index((k, eps, mu))
+            check(results[i], gs[k], cnts[k], eps, mu)
+
+
+def test_serve_threads(ctx, graphs):
+    gs, cnts = graphs
+    with serve.Server(ctx, max_batch=4, streams=2, slots=16) as sv:
+        gids = [sv.add_graph(g) for g in gs]
+        out = {}
+        errors = []
+
+        def client(c):
+            try:
+                for i in range(12):
+                    k = (c + i) % len(gs)
+                    eps, mu = QUERIES[(3 * c + i) % len(QUERIES)]
+                    out[(c, i)] = (k, eps, mu, sv.query(gids[k], eps, mu))
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        th = [threading.Thread(target=client, args=(c,)) for c in range(6)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errors, errors
+        for k, eps, mu, res in out.values():
+            check(res, gs[k], cnts[k], eps, mu)
+
+
+def test_serve_eviction(ctx, graphs):
+    """A budget that holds one of two equal-sized graphs at a time (plus one
+    result slot): alternating queries evict and re-admit them from the
+    server's host copies, and every answer stays exact."""
+    gs, cnts = graphs
+    pair = [gs[3], G.gen_rmat(14, 16, seed=65)]
+    pair_cnt = [cnts[3], oracle.cn_merge(pair[1])]
+    with serve.Server(ctx, slots=1) as probe:
+        gid = probe.add_graph(pair[1])
+        probe.query(gid, "0.5", 5)
+        foot = max(probe.graph_info(gid)["device_bytes"], 1)
+        slot = probe.stats()["slot_bytes"]
+    budget = int(1.6 * foot) + slot + (1 << 16)
+    with serve.Server(ctx, hbm_budget=budget, max_batch=2, streams=1, slots=1) as sv:
+        gids = [sv.add_graph(g) for g in pair]
+        for i in range(6):
+            k = i % 2
+            eps, mu = QUERIES[i]
+            check(sv.query(gids[k], eps, mu), pair[k], pair_cnt[k], eps, mu)
+        st = sv.stats()
+        print(f"[serve eviction] budget={budget} {st}", flush=True)
+        assert st["evictions"] >= 1 and st["admissions"] >= 3
+        assert st["graph_bytes"] + st["slot_bytes"] <= budget
+
+
+def test_serve_errors(ctx, graphs):
+    gs, _ = graphs
+    with serve.Server(ctx, hbm_budget=4 << 20, slots=2) as sv:
+        with pytest.raises(_ffi.AccError) as e:
+            sv.add_graph(gs[3])  # larger than the whole budget
+        assert e.value.code == -6
+        gid = sv.add_graph(gs[4])
+        with pytest.raises(_ffi.AccError) as e:
+            sv.submit(gid + 100, "0.5", 2)
+        assert e.value.code == -7
+        with pytest.raises(_ffi.AccError):
+            sv.submit(gid, (-1, 2), 2)
+        with pytest.raises(_ffi.AccError) as e:
+            sv.wait(12345, 0)
+        assert e.value.code == -7
+        t = sv.submit(gid, "0.6", 2)
+        sv.wait(t)
+        r = sv.result(t)
+        assert list(r["label"]) == [1, 1, 1, 1, 0] and r["core_num"] == 4
+        sv.release(t)
+        with pytest.raises(_ffi.AccError):
+            sv.result(t)
+        # a malformed graph is rejected at registration
+        bad = G.CSRGraph(np.array([0, 2, 4, 6], np.uint32), np.array([2, 1, 0, 2, 0, 1], np.int32))
+        with pytest.raises(_ffi.AccError) as e:
+            sv.add_graph(bad)
+        assert e.value.code == -2
+        sv.remove_graph(gid)
+        with pytest.raises(_ffi.AccError):
+            sv.submit(gid, "0.5", 2)
+
+
+def test_serve_cancel_pending(ctx, graphs):
+    gs, cnts = graphs
+    with serve.Server(ctx, max_batch=1, streams=1, slots=1) as sv:
+        gid = sv.add_graph(gs[3])
+        ts = [sv.submit(gid, "0.3", 5) for _ in range(8)]
+        # the last ones are still pending behind the single slot: cancel one
+        sv.release(ts[-1])
+        for t in ts[:-1]:
+            sv.wait(t)
+            check(sv.result(t), gs[3], cnts[3], "0.3", 5)
+            sv.release(t)
+        st = sv.stats()
+        assert st["cancelled"] + st["completed"] == 8
