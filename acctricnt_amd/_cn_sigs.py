@@ -14,14 +14,14 @@ vp = C.c_void_p
 class ServerConfig(C.Structure):
     """acc_server_config"""
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_int32), ("streams", C.c_int32),
-                ("slots", C.c_int32), ("reserved", C.c_int32)]
+                ("slots", C.c_int32), ("no_index", C.c_int32)]
 
 
 class ServerStats(C.Structure):
     """acc_server_stats"""
     _fields_ = [(name, C.c_uint64) for name in (
         "submitted", "completed", "failed", "cancelled", "batches", "batched_queries", "max_batch_seen",
-        "admissions", "evictions", "graph_bytes", "slot_bytes", "resident_graphs", "pending")]
+        "admissions", "evictions", "graph_bytes", "slot_bytes", "resident_graphs", "pending", "index_queries")]
 
 
 # name -> (restype, argtypes); kept in step with include/acc_cn.h

@@ -69,6 +69,11 @@ struct acc_graph {
   int32_t *d_flags = nullptr;           // validation flags
   int32_t *d_src = nullptr;             // source vertex of every slot (built for SCAN)
   int32_t *d_old = nullptr;             // caller vertex ids for cluster ids (NULL: identity)
+  // similarity index (built for serving): every vertex's slots sorted by a
+  // monotone float key of cn^2 / ((d_u+1)(d_v+1)), descending
+  float *d_skey = nullptr;     // sorted keys
+  int32_t *d_snbr = nullptr;   // neighbor of every sorted position
+  uint32_t *d_sslot = nullptr; // original slot of every sorted position
   bool cnt_plus2 = true;                // counts stored as 2 + |N(u) ∩ N(v)|
   // forward plan, binned by degree skew
   FwdUnit *d_fwd_wave = nullptr;  // few upper slots, small N<(b): one wave per source
@@ -135,6 +140,10 @@ struct ScanJob {
   const int32_t *cnt;
   const int32_t *old;  // caller vertex ids, or NULL
   uint8_t *sim;
+  const float *skey;     // similarity index (NULL: per-slot kernels)
+  const int32_t *snbr;
+  const uint32_t *sslot;
+  uint32_t *send;        // per vertex: end of the possibly-similar prefix
   int32_t *core_cnt;
   int32_t *label;
   int32_t *parent;
@@ -157,6 +166,7 @@ struct ScanJob {
 // Per-query device buffers, grown on demand.
 struct ScanBuffers {
   uint8_t *sim = nullptr;
+  uint32_t *send = nullptr;
   int32_t *core_cnt = nullptr;
   int32_t *label = nullptr;
   int32_t *parent = nullptr;
@@ -180,6 +190,13 @@ void plan_jobs(ScanJob *jobs, int nj, int cus, uint32_t *vblocks, uint32_t *sblo
 // Enqueues the eight SCAN kernels over the nj jobs at d_jobs (plan_jobs'd copy).
 int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
 int launch_pairs(const ScanJob *d_jobs, int nj, uint32_t sblocks, hipStream_t s);
+// Builds the similarity index of a resident graph (counts computed, slot
+// sources built); enqueued on the graph's context stream.
+int build_index(acc_graph *g);
+// Whether the index decides eps = p/q exactly (p, q < 2^26) on graph g.
+bool index_usable(const acc_graph *g, int64_t p, int64_t q);
+// The six index-path SCAN kernels over nj jobs whose skey is set.
+int launch_index_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
 }  // namespace accscan
 
 // Kernel launchers (k_forward.hip).

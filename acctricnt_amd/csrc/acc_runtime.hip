@@ -36,7 +36,8 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_fwd_big), static_cast<void *>(g->d_src),
                   static_cast<void *>(g->d_old), static_cast<void *>(g->d_edge_units),
                   static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
-                  static_cast<void *>(g->d_bmp_pool)})
+                  static_cast<void *>(g->d_bmp_pool), static_cast<void *>(g->d_skey),
+                  static_cast<void *>(g->d_snbr), static_cast<void *>(g->d_sslot)})
     if (p) (void)hipFree(p);
   if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);

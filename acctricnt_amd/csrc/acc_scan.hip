@@ -16,11 +16,19 @@
 #include <new>
 #include <type_traits>
 
+#include <hipcub/hipcub.hpp>
+
 #include "acc_internal.h"
 
 namespace {
 
 constexpr int kScanWG = 256;
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 __device__ __forceinline__ int32_t uf_load(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -256,6 +264,163 @@ __global__ __launch_bounds__(256) void k_sj_pairs(const ScanJob *__restrict__ jo
   }
 }
 
+// ---------------------------------------------------------------------------
+// Similarity index.  key(x) = float(double(x)) of x = cn^2 / ((d_u+1)(d_v+1)),
+// computed as a correctly rounded double division of two exact doubles
+// (cn < 2^26 and degrees below 2^26 keep both exact) and a round-to-nearest
+// conversion: a composition of monotone roundings, so key is monotone
+// nondecreasing in x.  With e = p^2 / q^2 (p, q < 2^26) and ke = key(e):
+// key(x) > ke implies x >= e (similar), key(x) < ke implies x < e (not
+// similar), and key(x) == ke is decided by the exact integer test.  Every
+// vertex's slots are sorted by key, descending, once per graph; a query then
+// finds its similar neighbors as a prefix of that order.
+__device__ __forceinline__ float sim_key(unsigned long long num, unsigned long long den) {
+  return static_cast<float>(static_cast<double>(num) / static_cast<double>(den));
+}
+
+__global__ __launch_bounds__(256) void k_idx_key(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                 const int32_t *__restrict__ src, const int32_t *__restrict__ cnt,
+                                                 uint64_t m, int32_t add, float *__restrict__ key,
+                                                 uint32_t *__restrict__ slot) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += gridDim.x * 256ull) {
+    const int32_t u = src[i], v = adj[i];
+    const unsigned long long c = static_cast<unsigned long long>(cnt[i] + add);
+    const unsigned long long d = static_cast<unsigned long long>(off[u + 1] - off[u] + 1) *
+                                 static_cast<unsigned long long>(off[v + 1] - off[v] + 1);
+    key[i] = sim_key(c * c, d);
+    slot[i] = static_cast<uint32_t>(i);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_idx_gather(const int32_t *__restrict__ adj,
+                                                    const uint32_t *__restrict__ slot, uint64_t m,
+                                                    int32_t *__restrict__ nbr) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < m; i += gridDim.x * 256ull) nbr[i] = adj[slot[i]];
+}
+
+// Exact similarity of slot s = (u, v): q^2 cn^2 >= p^2 (d_u+1)(d_v+1).
+__device__ __forceinline__ bool exact_sim(const ScanJob &J, uint32_t s, uint64_t du1, int32_t v) {
+  const uint64_t c = static_cast<uint64_t>(J.cnt[s] + J.add);
+  const uint64_t dv1 = static_cast<uint64_t>(J.off[v + 1] - J.off[v]) + 1;
+  const unsigned __int128 lhs = static_cast<unsigned __int128>(J.q2) * (c * c);
+  const unsigned __int128 rhs = static_cast<unsigned __int128>(J.p2) * (du1 * dv1);
+  return lhs >= rhs;
+}
+
+__global__ __launch_bounds__(64) void k_sx_zero(const ScanJob *__restrict__ jobs) {
+  if (threadIdx.x < 8) jobs[blockIdx.x].counters[threadIdx.x] = 0;
+}
+
+// First position in [lo, hi) of the descending keys K whose key is < ke
+// (strict) or <= ke (!strict).
+__device__ __forceinline__ uint32_t key_bound(const float *K, uint32_t lo, uint32_t hi, float ke, bool strict) {
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const float k = K[mid];
+    if (strict ? k >= ke : k > ke) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Core check from the index, with ke = key(eps^2) and K = u's sorted keys:
+// the mu-th largest key decides unless it ties with ke (K[mu-1] > ke: at
+// least mu similar neighbors; < ke: fewer than mu), in which case the tie
+// block takes the exact test.  send[u] = the end of a core's possibly-similar
+// prefix (keys >= ke), 0 for non-cores.  Degree sums over cores and
+// non-cores go to counters[5], counters[6].
+__global__ __launch_bounds__(256) void k_sx_core(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const float ke = sim_key(J.p2, J.q2);
+  unsigned long long core_deg = 0, other_deg = 0;
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
+    const uint32_t base = J.off[u];
+    const uint32_t d = J.off[u + 1] - base;
+    const float *K = J.skey + base;
+    bool core;
+    uint32_t end = 0;
+    int32_t count = 0;
+    if (J.mu <= 0) {
+      core = true;
+      end = key_bound(K, 0, d, ke, true);
+    } else if (d < static_cast<uint32_t>(J.mu)) {
+      core = false;
+    } else {
+      const uint32_t t = static_cast<uint32_t>(J.mu) - 1;
+      const float k = K[t];
+      if (k > ke) {
+        core = true;
+        end = key_bound(K, t + 1, d, ke, true);
+      } else if (k < ke) {
+        core = false;
+      } else {
+        const uint32_t lo = key_bound(K, 0, t, ke, false);
+        end = key_bound(K, t + 1, d, ke, true);
+        count = static_cast<int32_t>(lo);
+        for (uint32_t i = lo; i < end; i++)
+          if (exact_sim(J, J.sslot[base + i], static_cast<uint64_t>(d) + 1, J.snbr[base + i])) count++;
+        core = count >= J.mu;
+      }
+    }
+    J.core_cnt[u] = count;
+    J.label[u] = core ? 1 : 0;
+    J.parent[u] = static_cast<int32_t>(u);
+    J.rmin[u] = INT32_MAX;
+    J.rmax[u] = -1;
+    J.cl[u] = INT32_MAX;
+    J.send[u] = core ? end : 0;
+    if (core) core_deg += d; else other_deg += d;
+  }
+  core_deg = wave_sum_u64(core_deg);
+  other_deg = wave_sum_u64(other_deg);
+  if ((threadIdx.x & 63) == 0) {
+    if (core_deg) atomicAdd(&J.counters[5], core_deg);
+    if (other_deg) atomicAdd(&J.counters[6], other_deg);
+  }
+}
+
+// ClusterCore from the index, one thread per sorted position: position i of
+// core a's possibly-similar prefix unites a with its neighbor v when v > a is
+// a core and the pair is similar (exact test on a key tie).
+__global__ __launch_bounds__(256) void k_sx_union(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const float ke = sim_key(J.p2, J.q2);
+  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
+    const int32_t a = J.src[i];
+    if (J.label[a] != 1) continue;
+    const uint32_t base = J.off[a];
+    if (static_cast<uint32_t>(i) - base >= J.send[a]) continue;
+    const int32_t v = J.snbr[i];
+    if (v <= a || J.label[v] != 1) continue;
+    if (J.skey[i] == ke &&
+        !exact_sim(J, J.sslot[i], static_cast<uint64_t>(J.off[a + 1] - base) + 1, v))
+      continue;
+    uf_union(J.parent, a, v);
+  }
+}
+
+// CheckHub, one thread per slot, from the side with fewer adjacency slots:
+// cores push their root to non-core neighbors, or non-cores pull their core
+// neighbors' roots; both leave the smallest and largest root in rmin / rmax.
+__global__ __launch_bounds__(256) void k_sx_hub(const ScanJob *__restrict__ jobs, int nj) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const bool push = J.counters[5] <= J.counters[6];
+  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
+    const int32_t a = J.src[i];
+    const bool a_core = J.label[a] == 1;
+    if (a_core != push) continue;
+    const int32_t w = J.adj[i];
+    const bool w_core = J.label[w] == 1;
+    if (w_core == push) continue;
+    const int32_t r = push ? J.parent[a] : J.parent[w];
+    const int32_t x = push ? w : a;
+    if (r < J.rmin[x]) atomicMin(&J.rmin[x], r);
+    if (r > J.rmax[x]) atomicMax(&J.rmax[x], r);
+  }
+}
+
 inline unsigned grid_for(uint64_t items, int cus) {
   const uint64_t want = (items + 255) / 256;
   const uint64_t c = static_cast<uint64_t>(cus > 0 ? cus : 256) * 8;
@@ -292,6 +457,85 @@ int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblock
   hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
   hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
   ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_index_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sx_zero, dim3(nj), dim3(64), 0, s, d_jobs);
+  hipLaunchKernelGGL(k_sx_core, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sx_union, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_compress, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sx_hub, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+bool index_usable(const acc_graph *g, int64_t p, int64_t q) {
+  const int64_t lim = int64_t(1) << 26;
+  return g->d_skey && p < lim && q < lim;
+}
+
+int build_index(acc_graph *g) {
+  if (g->d_skey) return ACC_OK;
+  if (!g->have_result) return acc_set_error(ACC_EINVAL, "index needs the graph's counts");
+  if (g->max_deg + 2 >= (1u << 26)) return acc_set_error(ACC_EINVAL, "degrees too large for the similarity index");
+  ACC_TRY(acc_graph_ensure_src(g));
+  const uint64_t m = g->m;
+  if (m >= (1ull << 31)) return acc_set_error(ACC_EINVAL, "too many slots for the similarity index");
+  hipStream_t st = g->ctx->stream;
+  const int cus = g->ctx->num_cus;
+  float *key_in = nullptr;
+  uint32_t *slot_in = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    if (key_in) (void)hipFree(key_in);
+    if (slot_in) (void)hipFree(slot_in);
+    if (temp) (void)hipFree(temp);
+  };
+  auto fail = [&](const char *what, hipError_t e) {
+    cleanup();
+    for (void **p : {reinterpret_cast<void **>(&g->d_skey), reinterpret_cast<void **>(&g->d_snbr),
+                     reinterpret_cast<void **>(&g->d_sslot)})
+      if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+      }
+    return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
+                         std::string(what) + ": " + hipGetErrorString(e));
+  };
+  const size_t mm = m ? m : 1;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&key_in), sizeof(float) * mm);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&slot_in), sizeof(uint32_t) * mm);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_skey), sizeof(float) * mm);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_sslot), sizeof(uint32_t) * mm);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_snbr), sizeof(int32_t) * mm);
+  if (e != hipSuccess) return fail("index allocation", e);
+  if (m) {
+    hipLaunchKernelGGL(k_idx_key, dim3(grid_for(m, cus)), dim3(256), 0, st, g->d_off, g->d_adj, g->d_src, g->d_cnt,
+                       m, g->cnt_plus2 ? 0 : 2, key_in, slot_in);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail("index keys", e);
+    e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(nullptr, temp_bytes, key_in, g->d_skey, slot_in,
+                                                              g->d_sslot, static_cast<int>(m),
+                                                              static_cast<int>(g->n), g->d_off, g->d_off + 1, 0,
+                                                              32, st);
+    if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 1);
+    if (e == hipSuccess)
+      e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(temp, temp_bytes, key_in, g->d_skey, slot_in,
+                                                                g->d_sslot, static_cast<int>(m),
+                                                                static_cast<int>(g->n), g->d_off, g->d_off + 1, 0,
+                                                                32, st);
+    if (e != hipSuccess) return fail("index sort", e);
+    hipLaunchKernelGGL(k_idx_gather, dim3(grid_for(m, cus)), dim3(256), 0, st, g->d_adj, g->d_sslot, m, g->d_snbr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return fail("index gather", e);
+  }
+  cleanup();
+  g->device_bytes += 12 * mm;
   return ACC_OK;
 }
 
@@ -339,7 +583,8 @@ void free_ws(acc_scan_ws *w) {
 namespace accscan {
 
 void free_buffers(ScanBuffers *b) {
-  for (void *p : {static_cast<void *>(b->sim), static_cast<void *>(b->core_cnt), static_cast<void *>(b->label),
+  for (void *p : {static_cast<void *>(b->sim), static_cast<void *>(b->send), static_cast<void *>(b->core_cnt),
+                  static_cast<void *>(b->label),
                   static_cast<void *>(b->parent), static_cast<void *>(b->rmin), static_cast<void *>(b->rmax),
                   static_cast<void *>(b->cl), static_cast<void *>(b->cid), static_cast<void *>(b->counters)})
     if (p) (void)hipFree(p);
@@ -364,6 +609,7 @@ int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m) {
   };
   int rc = ACC_OK;
   if (!rc) rc = alloc(&b->sim, mc);
+  if (!rc) rc = alloc(&b->send, nc);
   if (!rc) rc = alloc(&b->core_cnt, nc);
   if (!rc) rc = alloc(&b->label, nc);
   if (!rc) rc = alloc(&b->parent, nc);
@@ -383,7 +629,7 @@ int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m) {
 
 uint64_t buffer_bytes(uint32_t n, uint64_t m) {
   const uint64_t nn = n ? n : 1, mm = m ? m : 1;
-  return mm + 8 * sizeof(int32_t) * nn + 8 * sizeof(unsigned long long);
+  return mm + 9 * sizeof(int32_t) * nn + 8 * sizeof(unsigned long long);
 }
 
 int check_eps(int64_t eps_num, int64_t eps_den) {
@@ -401,6 +647,10 @@ ScanJob make_job(const acc_graph *g, const ScanBuffers &b, int64_t eps_num, int6
   j.cnt = g->d_cnt;
   j.old = g->d_old;
   j.sim = b.sim;
+  j.skey = g->d_skey;
+  j.snbr = g->d_snbr;
+  j.sslot = g->d_sslot;
+  j.send = b.send;
   j.core_cnt = b.core_cnt;
   j.label = b.label;
   j.parent = b.parent;

@@ -80,8 +80,9 @@ struct Batch {
 };
 
 uint64_t graph_estimate(uint32_t n, uint64_t m) {
-  // offsets, adjacency, counts, reverse slots, slot sources, work plans
-  return 4ull * (n + 1) + 4ull * (m + 64) + 4ull * (m + 4) * 2 + 4ull * m + 16ull * (n + m / 64 + 16) + (1ull << 20);
+  // offsets, adjacency, counts, reverse slots, slot sources, similarity index, work plans
+  return 4ull * (n + 1) + 4ull * (m + 64) + 4ull * (m + 4) * 2 + 4ull * m + 12ull * m + 16ull * (n + m / 64 + 16) +
+         (1ull << 20);
 }
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
@@ -155,6 +156,8 @@ int upload_graph(acc_server *s, ServedGraph &sg) {
   if (!rc) rc = acc_cn_compute(g, ACC_PATH_AUTO, 0);
   if (!rc) rc = acc_graph_ensure_src(g);
   if (!rc) rc = acc_ctx_synchronize(s->ctx);
+  // the index is an optimisation: queries it cannot decide use the per-slot kernels
+  if (!rc && !s->cfg.no_index && accscan::build_index(g) == ACC_OK) rc = acc_ctx_synchronize(s->ctx);
   if (rc) {
     if (g) acc_graph_release(g);
     return rc;
@@ -334,22 +337,44 @@ bool launch_batch(acc_server *s) {
   }
   s->pending.erase(std::remove(s->pending.begin(), s->pending.end(), 0ull), s->pending.end());
   if (take.empty()) return false;
+  // queries the similarity index decides go first in the job table, the
+  // rest (eps beyond the index's range, graphs without an index) after
   const int nj = static_cast<int>(take.size());
+  {
+    // each slot was grown for its own query's graph: keep the pairs together
+    std::vector<std::pair<uint64_t, int>> ts;
+    for (int j = 0; j < nj; j++) ts.emplace_back(take[j], slot_of[j]);
+    std::stable_partition(ts.begin(), ts.end(), [&](const std::pair<uint64_t, int> &x) {
+      const Query &q = s->queries[x.first];
+      return accscan::index_usable(s->graphs[q.gid].g, q.p, q.q);
+    });
+    for (int j = 0; j < nj; j++) {
+      take[j] = ts[j].first;
+      slot_of[j] = ts[j].second;
+    }
+  }
+  int ni = 0;
   for (int j = 0; j < nj; j++) {
     Query &q = s->queries[take[j]];
     ServedGraph &sg = s->graphs[q.gid];
     sg.waiting--;
     sg.running++;
     sg.last_use = ++s->clock;
-    q.slot = slot_of[j];
     q.state = kRunning;
-    b->h_jobs[j] = accscan::make_job(sg.g, s->slots[q.slot].buf, q.p, q.q, q.mu);
+    if (accscan::index_usable(sg.g, q.p, q.q)) ni++;
   }
-  uint32_t vblocks = 0, sblocks = 0;
-  accscan::plan_jobs(b->h_jobs, nj, s->ctx->num_cus, &vblocks, &sblocks);
+  for (int j = 0; j < nj; j++) {
+    Query &q = s->queries[take[j]];
+    q.slot = slot_of[j];
+    b->h_jobs[j] = accscan::make_job(s->graphs[q.gid].g, s->slots[q.slot].buf, q.p, q.q, q.mu);
+  }
+  uint32_t vb_i = 0, sb_i = 0, vblocks = 0, sblocks = 0;
+  accscan::plan_jobs(b->h_jobs, ni, s->ctx->num_cus, &vb_i, &sb_i);
+  accscan::plan_jobs(b->h_jobs + ni, nj - ni, s->ctx->num_cus, &vblocks, &sblocks);
   hipError_t e = hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(ScanJob) * nj, hipMemcpyHostToDevice, b->stream);
-  int rc = e == hipSuccess ? accscan::launch_jobs(b->d_jobs, nj, vblocks, sblocks, b->stream)
-                           : acc_set_error(ACC_EDEVICE, std::string("job upload: ") + hipGetErrorString(e));
+  int rc = e == hipSuccess ? ACC_OK : acc_set_error(ACC_EDEVICE, std::string("job upload: ") + hipGetErrorString(e));
+  if (!rc && ni) rc = accscan::launch_index_jobs(b->d_jobs, ni, vb_i, sb_i, b->stream);
+  if (!rc && nj > ni) rc = accscan::launch_jobs(b->d_jobs + ni, nj - ni, vblocks, sblocks, b->stream);
   for (int j = 0; j < nj && !rc; j++) {
     const Query &q = s->queries[take[j]];
     const ServedGraph &sg = s->graphs[q.gid];
@@ -377,6 +402,7 @@ bool launch_batch(acc_server *s) {
   b->busy = true;
   b->tickets = take;
   s->st.batches++;
+  s->st.index_queries += static_cast<uint64_t>(ni);
   s->st.batched_queries += static_cast<uint64_t>(nj);
   s->st.max_batch_seen = std::max<uint64_t>(s->st.max_batch_seen, static_cast<uint64_t>(nj));
   return true;
@@ -432,6 +458,7 @@ void acc_server_config_default(acc_server_config *cfg) {
   cfg->max_batch = 16;
   cfg->streams = 4;
   cfg->slots = 32;
+  cfg->no_index = 0;
 }
 
 int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **out) {

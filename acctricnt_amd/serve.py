@@ -36,7 +36,7 @@ def eps_fraction(eps) -> Fraction:
 
 class Server:
     def __init__(self, ctx: Context, hbm_budget: int | None = None, max_batch: int | None = None,
-                 streams: int | None = None, slots: int | None = None):
+                 streams: int | None = None, slots: int | None = None, index: bool = True):
         self.ctx = ctx
         self._lib = ctx._lib
         cfg = ServerConfig()
@@ -49,7 +49,8 @@ class Server:
             cfg.streams = int(streams)
         if slots is not None:
             cfg.slots = int(slots)
-        self.config = {k: getattr(cfg, k) for k in ("hbm_budget", "max_batch", "streams", "slots")}
+        cfg.no_index = 0 if index else 1
+        self.config = {k: getattr(cfg, k) for k in ("hbm_budget", "max_batch", "streams", "slots", "no_index")}
         h = C.c_void_p()
         check_cn(self._lib.acc_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self._h = h

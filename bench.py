@@ -242,15 +242,21 @@ def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16
             sv.release(t)
 
     before = sv.stats()
+    # the submitting loop sleeps between arrivals (a spin would hold the
+    # interpreter lock and starve the collecting thread that frees slots)
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(1e-4)
     th = threading.Thread(target=collect)
     th.start()
     start = time.perf_counter()
     for i, (k, eps, mu) in enumerate(plan):
-        while time.perf_counter() - start < arrivals[i]:
-            pass
+        wait = arrivals[i] - (time.perf_counter() - start)
+        if wait > 0:
+            time.sleep(wait)
         tickets[i] = sv.submit(gids[k], eps, mu)
         ready.release()
     th.join()
+    sys.setswitchinterval(switch)
     wall = time.perf_counter() - start
     after = sv.stats()
     same = True

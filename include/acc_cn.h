@@ -171,7 +171,11 @@ int acc_scan_pairs(acc_scan_ws *ws, int32_t *pairs, uint64_t cap, uint64_t *coun
  * SCAN queries (graph, eps = eps_num/eps_den, mu) submitted from any thread.
  * A worker thread batches pending queries across graphs into one set of SCAN
  * launches per batch, on one of `streams` streams, while earlier batches are
- * still running (continuous batching).  Each query's labels (0 unclassified,
+ * still running (continuous batching).  At admission the server also builds
+ * a similarity index per graph (every vertex's slots sorted by a monotone
+ * key of cn^2 / ((d_u+1)(d_v+1))), with which a query's core check is O(1)
+ * per vertex and its clustering visits only similar slots; ties with eps
+ * take the exact integer test, so results do not change.  Each query's labels (0 unclassified,
  * 1 core, 2 hub, as acc_scan_results) and cluster ids (-1 for non-cores) are
  * the ones acc_scan_run gives for the same graph and parameters alone; they
  * are copied to pinned host memory, readable with acc_server_result until
@@ -183,13 +187,14 @@ typedef struct acc_server_config {
   int32_t max_batch;   /* queries per batched launch, 1..256 */
   int32_t streams;     /* batches in flight at once, 1..16 */
   int32_t slots;       /* queries holding results at once (launched, not released), 1..4096 */
-  int32_t reserved;
+  int32_t no_index;    /* 1: build no similarity index (every query takes the per-slot kernels) */
 } acc_server_config;
 typedef struct acc_server_stats {
   uint64_t submitted, completed, failed, cancelled;
   uint64_t batches, batched_queries, max_batch_seen;
   uint64_t admissions, evictions;
   uint64_t graph_bytes, slot_bytes, resident_graphs, pending;
+  uint64_t index_queries; /* queries decided through the similarity index */
 } acc_server_stats;
 
 #define ACC_QUERY_PENDING 0

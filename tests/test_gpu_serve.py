@@ -45,10 +45,11 @@ def check(res, g, cnt, eps, mu):
         assert res[k] == ref[k], k
 
 
-def test_serve_matches_oracle_and_alone(ctx, graphs):
+@pytest.mark.parametrize("index", [True, False])
+def test_serve_matches_oracle_and_alone(ctx, graphs, index):
     gs, cnts = graphs
     rng = np.random.default_rng(7)
-    with serve.Server(ctx, max_batch=8, streams=3, slots=48) as sv:
+    with serve.Server(ctx, max_batch=8, streams=3, slots=48, index=index) as sv:
         gids = [sv.add_graph(g) for g in gs]
         # the mix: all queries submitted at once, batched across graphs
         mix = [(int(rng.integers(len(gs))), *QUERIES[int(rng.integers(len(QUERIES)))]) for _ in range(120)]
@@ -62,6 +63,7 @@ def test_serve_matches_oracle_and_alone(ctx, graphs):
         print(f"[serve] {st}", flush=True)
         assert st["completed"] == len(mix) and st["failed"] == 0
         assert st["max_batch_seen"] > 1, "queries were never batched"
+        assert st["index_queries"] == (len(mix) if index else 0)
         # the same queries alone, through a workspace
         alone = {}
         for k, eps, mu in set(mix):
@@ -173,3 +175,37 @@ def test_serve_cancel_pending(ctx, graphs):
             sv.release(t)
         st = sv.stats()
         assert st["cancelled"] + st["completed"] == 8
+
+
+def cube():
+    s = [u for u in range(8) for b in range(3) if u < u ^ (1 << b)]
+    d = [u ^ (1 << b) for u in range(8) for b in range(3) if u < u ^ (1 << b)]
+    return G.from_edges(8, s, d, name="Q3")
+
+
+def test_serve_index_ties_and_fallback(ctx):
+    """Similarities equal to eps take the exact test: the cube Q3 has
+    cn^2 / D = 4/16 on every edge, so eps = 0.5 makes every edge similar and
+    eps = 0.51 none; K_7 has similarity 1 everywhere (eps = 1).  An eps whose
+    numerator and denominator exceed 2^26 is outside the index and takes the
+    per-slot kernels in the same batch."""
+    from fractions import Fraction
+
+    k7 = G.from_edges(7, *np.triu_indices(7, 1))
+    rm = G.gen_rmat(12, 8, seed=66)
+    cases = [(cube(), "0.5", 3), (cube(), "0.51", 1), (cube(), "0.5", 4), (k7, "1", 6), (k7, "1", 7),
+             (rm, Fraction(123456789, 987654321), 2), (rm, "0.3", 2), (rm, Fraction(67108865, 134217729), 3)]
+    with serve.Server(ctx, max_batch=8, streams=1, slots=8) as sv:
+        ids = {}
+        for g, _, _ in cases:
+            if id(g) not in ids:
+                ids[id(g)] = sv.add_graph(g)
+        ts = [sv.submit(ids[id(g)], eps, mu) for g, eps, mu in cases]
+        for (g, eps, mu), t in zip(cases, ts):
+            sv.wait(t)
+            check(sv.result(t), g, oracle.cn_merge(g), eps, mu)
+            sv.release(t)
+        st = sv.stats()
+        assert st["index_queries"] == len(cases) - 2
+    assert oracle.scan(cube(), oracle.cn_merge(cube()), "0.5", 3)["core_num"] == 8
+    assert oracle.scan(cube(), oracle.cn_merge(cube()), "0.51", 1)["core_num"] == 0
