@@ -65,7 +65,8 @@ SIGNATURES = {
     "acc_server_submit": (C.c_int, [vp, C.c_int32, C.c_int64, C.c_int64, C.c_int32, u64p]),
     "acc_server_wait": (C.c_int, [vp, C.c_uint64, C.c_int32]),
     "acc_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(C.c_int)]),
-    "acc_server_result": (C.c_int, [vp, C.c_uint64, C.POINTER(i32p), C.POINTER(i32p), u32p, C.POINTER(C.c_int64),
+    "acc_server_result": (C.c_int, [vp, C.c_uint64, C.POINTER(C.POINTER(C.c_int8)), C.POINTER(i32p), u32p,
+                                    C.POINTER(C.c_int64),
                                     C.POINTER(C.c_double)]),
     "acc_server_release": (C.c_int, [vp, C.c_uint64]),
     "acc_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),

@@ -151,7 +151,10 @@ struct ScanJob {
   int32_t *rmax;
   int32_t *cl;
   int32_t *cid;
-  unsigned long long *counters;  // core, cluster, hub, outlier, pairs
+  unsigned long long *counters;  // core, cluster, hub, outlier, pairs, index-path sums (5..7)
+  int8_t *out_label;             // host-mapped copies of the results (NULL: none)
+  int32_t *out_cid;
+  unsigned long long *out_counters;
   int2 *pairs;
   unsigned long long pairs_cap;
   unsigned long long p2, q2;

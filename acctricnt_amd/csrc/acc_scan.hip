@@ -45,6 +45,17 @@ __device__ __forceinline__ int32_t uf_find(int32_t *parent, int32_t x) {
   }
 }
 
+// Root of x without writes.  Used once the unions are done: a path-halving
+// store racing with another thread's store of a root could leave a pointer
+// to a non-root ancestor behind.
+__device__ __forceinline__ int32_t uf_root(const int32_t *parent, int32_t x) {
+  for (;;) {
+    const int32_t p = uf_load(&parent[x]);
+    if (p == x) return x;
+    x = p;
+  }
+}
+
 // Links the sets of a and b, the larger root under the smaller.
 __device__ __forceinline__ void uf_union(int32_t *parent, int32_t a, int32_t b) {
   for (;;) {
@@ -168,12 +179,14 @@ __global__ __launch_bounds__(256) void k_sj_union(const ScanJob *__restrict__ jo
   }
 }
 
+// Points every core at its root.  Only roots are stored, and every pointer
+// leads to the root, so concurrent walks stay correct.
 __global__ __launch_bounds__(256) void k_sj_compress(const ScanJob *__restrict__ jobs, int nj) {
   uint32_t lb, nb;
   const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u)
     if (J.label[u] == 1)
-      __hip_atomic_store(&J.parent[u], uf_find(J.parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
+      __hip_atomic_store(&J.parent[u], uf_root(J.parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -235,9 +248,22 @@ __global__ __launch_bounds__(256) void k_sj_finalize(const ScanJob *__restrict__
   }
 }
 
+// Cluster ids; with host-mapped outputs set, the labels (as int8), cluster
+// ids and the four counters go straight to host memory instead.
 __global__ __launch_bounds__(256) void k_sj_cid(const ScanJob *__restrict__ jobs, int nj) {
   uint32_t lb, nb;
   const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  if (J.out_cid) {
+    for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
+      const int32_t l = J.label[u];
+      J.out_cid[u] = l == 1 ? J.cl[J.parent[u]] : -1;
+      J.out_label[u] = static_cast<int8_t>(l);
+    }
+    if (lb == 0 && threadIdx.x < 4) J.out_counters[threadIdx.x] = J.counters[threadIdx.x];
+    // the host reads these once the batch's completion event fires
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    return;
+  }
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u)
     J.cid[u] = J.label[u] == 1 ? J.cl[J.parent[u]] : -1;
 }
@@ -323,6 +349,7 @@ __device__ __forceinline__ uint32_t key_bound(const float *K, uint32_t lo, uint3
 }
 
 // Core check from the index, with ke = key(eps^2) and K = u's sorted keys:
+// (counters[7] receives the total length of the cores' prefixes)
 // the mu-th largest key decides unless it ties with ke (K[mu-1] > ke: at
 // least mu similar neighbors; < ke: fewer than mu), in which case the tie
 // block takes the exact test.  send[u] = the end of a core's possibly-similar
@@ -332,7 +359,7 @@ __global__ __launch_bounds__(256) void k_sx_core(const ScanJob *__restrict__ job
   uint32_t lb, nb;
   const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
   const float ke = sim_key(J.p2, J.q2);
-  unsigned long long core_deg = 0, other_deg = 0;
+  unsigned long long core_deg = 0, other_deg = 0, prefix = 0;
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
     const uint32_t base = J.off[u];
     const uint32_t d = J.off[u + 1] - base;
@@ -369,51 +396,67 @@ __global__ __launch_bounds__(256) void k_sx_core(const ScanJob *__restrict__ job
     J.rmax[u] = -1;
     J.cl[u] = INT32_MAX;
     J.send[u] = core ? end : 0;
-    if (core) core_deg += d; else other_deg += d;
+    if (core) {
+      core_deg += d;
+      prefix += end;
+    } else {
+      other_deg += d;
+    }
   }
   core_deg = wave_sum_u64(core_deg);
   other_deg = wave_sum_u64(other_deg);
+  prefix = wave_sum_u64(prefix);
   if ((threadIdx.x & 63) == 0) {
     if (core_deg) atomicAdd(&J.counters[5], core_deg);
     if (other_deg) atomicAdd(&J.counters[6], other_deg);
+    if (prefix) atomicAdd(&J.counters[7], prefix);
   }
 }
 
-// ClusterCore from the index, one thread per sorted position: position i of
-// core a's possibly-similar prefix unites a with its neighbor v when v > a is
-// a core and the pair is similar (exact test on a key tie).
+// Union of core a with the neighbor at sorted position i of its
+// possibly-similar prefix, when that neighbor is a core v > a and the pair is
+// similar (exact test on a key tie).
+__device__ __forceinline__ void union_at(const ScanJob &J, float ke, int32_t a, uint32_t i) {
+  const int32_t v = J.snbr[i];
+  if (v <= a || J.label[v] != 1) return;
+  if (J.skey[i] == ke &&
+      !exact_sim(J, J.sslot[i], static_cast<uint64_t>(J.off[a + 1] - J.off[a]) + 1, v))
+    return;
+  uf_union(J.parent, a, v);
+}
+
+// ClusterCore from the index, one thread per sorted position (even load
+// whatever the degree skew), skipping positions outside a core's
+// possibly-similar prefix; nothing to do when the prefixes are empty
+// (counters[7]).
 __global__ __launch_bounds__(256) void k_sx_union(const ScanJob *__restrict__ jobs, int nj) {
   uint32_t lb, nb;
   const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  if (J.counters[7] == 0) return;
   const float ke = sim_key(J.p2, J.q2);
   for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
     const int32_t a = J.src[i];
     if (J.label[a] != 1) continue;
-    const uint32_t base = J.off[a];
-    if (static_cast<uint32_t>(i) - base >= J.send[a]) continue;
-    const int32_t v = J.snbr[i];
-    if (v <= a || J.label[v] != 1) continue;
-    if (J.skey[i] == ke &&
-        !exact_sim(J, J.sslot[i], static_cast<uint64_t>(J.off[a + 1] - base) + 1, v))
-      continue;
-    uf_union(J.parent, a, v);
+    if (static_cast<uint32_t>(i) - J.off[a] >= J.send[a]) continue;
+    union_at(J, ke, a, static_cast<uint32_t>(i));
   }
 }
 
-// CheckHub, one thread per slot, from the side with fewer adjacency slots:
-// cores push their root to non-core neighbors, or non-cores pull their core
-// neighbors' roots; both leave the smallest and largest root in rmin / rmax.
+// CheckHub, one thread per slot, from the side with fewer adjacency slots
+// (counters[5] over cores, counters[6] over non-cores): cores push their root
+// to non-core neighbors, or non-cores pull their core neighbors' roots; both
+// leave the smallest and largest root in rmin / rmax.  Nothing to do when
+// either side is empty.
 __global__ __launch_bounds__(256) void k_sx_hub(const ScanJob *__restrict__ jobs, int nj) {
   uint32_t lb, nb;
   const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
   const bool push = J.counters[5] <= J.counters[6];
+  if ((push ? J.counters[5] : J.counters[6]) == 0) return;
   for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
     const int32_t a = J.src[i];
-    const bool a_core = J.label[a] == 1;
-    if (a_core != push) continue;
+    if ((J.label[a] == 1) != push) continue;
     const int32_t w = J.adj[i];
-    const bool w_core = J.label[w] == 1;
-    if (w_core == push) continue;
+    if ((J.label[w] == 1) == push) continue;
     const int32_t r = push ? J.parent[a] : J.parent[w];
     const int32_t x = push ? w : a;
     if (r < J.rmin[x]) atomicMin(&J.rmin[x], r);

@@ -12,9 +12,10 @@
 // streams, so a batch is launched while earlier batches are still on the
 // device: arrivals are admitted continuously as result slots and streams
 // free up.  Each query owns a result slot (device buffers plus pinned host
-// copies of its labels and cluster ids) from launch until the caller
-// releases it.  Latency runs from submission to the worker observing the
-// batch's completion event, with the results already in host memory.
+// memory that the last kernel writes its labels, cluster ids and counters
+// into) from launch until the caller releases it.  One thread per stream
+// waits for that stream's batch; latency runs from submission to that
+// thread seeing the batch complete, with the results already in host memory.
 
 #include <algorithm>
 #include <chrono>
@@ -61,15 +62,25 @@ struct Query {
   int64_t counts[4] = {0, 0, 0, 0};
 };
 
+// Result slot: device buffers of one query plus pinned host memory that the
+// last SCAN kernel writes the results into (mapped into the device's address
+// space: d_* are the device-side addresses of h_*).  The host memory is
+// coherent, so the kernel's writes bypass the GPU caches and are all in host
+// memory once the batch's completion event has fired.
 struct Slot {
   ScanBuffers buf;
-  int32_t *h_label = nullptr;  // pinned
-  int32_t *h_cid = nullptr;    // pinned
+  int8_t *h_label = nullptr;
+  int32_t *h_cid = nullptr;
   unsigned long long *h_counters = nullptr;
+  int8_t *d_label = nullptr;
+  int32_t *d_cid = nullptr;
+  unsigned long long *d_counters = nullptr;
   uint32_t h_cap = 0;
   bool busy = false;
 };
 
+// One stream with at most one batch in flight, and the thread that waits for
+// the batch's completion event.
 struct Batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
@@ -77,6 +88,7 @@ struct Batch {
   ScanJob *h_jobs = nullptr;  // pinned staging
   std::vector<uint64_t> tickets;
   bool busy = false;
+  std::thread waiter;
 };
 
 uint64_t graph_estimate(uint32_t n, uint64_t m) {
@@ -104,6 +116,7 @@ struct acc_server {
   uint64_t next_ticket = 1;
   std::vector<Slot> slots;
   std::vector<Batch> batches;
+  std::condition_variable cv_batch;  // a batch was launched
   std::thread worker;
   bool stop = false;
   uint64_t clock = 0;
@@ -214,17 +227,22 @@ int reserve_slot(acc_server *s, Slot &sl, uint32_t n, uint64_t m) {
   if (n > sl.h_cap || !sl.h_counters) {
     if (sl.h_label) (void)hipHostFree(sl.h_label);
     if (sl.h_cid) (void)hipHostFree(sl.h_cid);
-    sl.h_label = sl.h_cid = nullptr;
+    sl.h_label = nullptr;
+    sl.h_cid = nullptr;
     sl.h_cap = 0;
-    const size_t bytes = sizeof(int32_t) * std::max<uint32_t>(n, 1);
-    if (hipHostMalloc(reinterpret_cast<void **>(&sl.h_label), bytes, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&sl.h_cid), bytes, hipHostMallocDefault) != hipSuccess)
+    const size_t k = std::max<uint32_t>(n, 1);
+    if (hipHostMalloc(reinterpret_cast<void **>(&sl.h_label), k, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&sl.h_cid), sizeof(int32_t) * k, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
       return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result buffers failed");
     if (!sl.h_counters &&
         hipHostMalloc(reinterpret_cast<void **>(&sl.h_counters), 8 * sizeof(unsigned long long),
-                      hipHostMallocDefault) != hipSuccess)
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
       return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result counters failed");
-    sl.h_cap = std::max<uint32_t>(n, 1);
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_label), sl.h_label, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_cid), sl.h_cid, 0) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_counters), sl.h_counters, 0) != hipSuccess)
+      return acc_set_error(ACC_EDEVICE, "device address of result buffers unavailable");
+    sl.h_cap = static_cast<uint32_t>(k);
   }
   return ACC_OK;
 }
@@ -237,36 +255,44 @@ void fail_query(acc_server *s, Query &q, int rc) {
   s->st.failed++;
 }
 
-// Marks finished batches' queries done.  Caller holds s->mu.
-bool reap(acc_server *s) {
-  bool any = false;
-  for (Batch &b : s->batches) {
-    if (!b.busy) continue;
-    const hipError_t e = hipEventQuery(b.done);
-    if (e == hipErrorNotReady) continue;
-    const Clock::time_point now = Clock::now();
-    for (uint64_t t : b.tickets) {
-      auto it = s->queries.find(t);
-      if (it == s->queries.end()) continue;
-      Query &q = it->second;
-      ServedGraph &sg = s->graphs[q.gid];
-      sg.running--;
-      if (e != hipSuccess) {
-        acc_set_error(ACC_EDEVICE, std::string("SCAN batch: ") + hipGetErrorString(e));
-        fail_query(s, q, ACC_EDEVICE);
-        continue;
-      }
-      const Slot &sl = s->slots[q.slot];
-      for (int i = 0; i < 4; i++) q.counts[i] = static_cast<int64_t>(sl.h_counters[i]);
-      q.state = kDone;
-      q.t_done = now;
-      s->st.completed++;
+// Marks a finished batch's queries done (e: its completion status).  Caller holds s->mu.
+void finish_batch(acc_server *s, Batch &b, hipError_t e) {
+  const Clock::time_point now = Clock::now();
+  for (uint64_t t : b.tickets) {
+    auto it = s->queries.find(t);
+    if (it == s->queries.end()) continue;
+    Query &q = it->second;
+    ServedGraph &sg = s->graphs[q.gid];
+    sg.running--;
+    if (e != hipSuccess) {
+      acc_set_error(ACC_EDEVICE, std::string("SCAN batch: ") + hipGetErrorString(e));
+      fail_query(s, q, ACC_EDEVICE);
+      continue;
     }
-    b.tickets.clear();
-    b.busy = false;
-    any = true;
+    const Slot &sl = s->slots[q.slot];
+    for (int i = 0; i < 4; i++) q.counts[i] = static_cast<int64_t>(sl.h_counters[i]);
+    q.state = kDone;
+    q.t_done = now;
+    s->st.completed++;
   }
-  return any;
+  b.tickets.clear();
+  b.busy = false;
+}
+
+// Completion thread of one stream: waits for each launched batch's event.
+void waiter_main(acc_server *s, Batch *b) {
+  (void)hipSetDevice(s->ctx->device);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_batch.wait(lk, [&]() { return s->stop || b->busy; });
+    if (!b->busy) return;  // stopping with nothing in flight
+    lk.unlock();
+    const hipError_t e = hipEventSynchronize(b->done);
+    lk.lock();
+    finish_batch(s, *b, e);
+    s->cv_done.notify_all();
+    s->cv_work.notify_one();
+  }
 }
 
 bool any_running(const acc_server *s) {
@@ -366,7 +392,12 @@ bool launch_batch(acc_server *s) {
   for (int j = 0; j < nj; j++) {
     Query &q = s->queries[take[j]];
     q.slot = slot_of[j];
-    b->h_jobs[j] = accscan::make_job(s->graphs[q.gid].g, s->slots[q.slot].buf, q.p, q.q, q.mu);
+    const Slot &sl = s->slots[q.slot];
+    ScanJob &job = b->h_jobs[j];
+    job = accscan::make_job(s->graphs[q.gid].g, sl.buf, q.p, q.q, q.mu);
+    job.out_label = sl.d_label;
+    job.out_cid = sl.d_cid;
+    job.out_counters = sl.d_counters;
   }
   uint32_t vb_i = 0, sb_i = 0, vblocks = 0, sblocks = 0;
   accscan::plan_jobs(b->h_jobs, ni, s->ctx->num_cus, &vb_i, &sb_i);
@@ -375,18 +406,6 @@ bool launch_batch(acc_server *s) {
   int rc = e == hipSuccess ? ACC_OK : acc_set_error(ACC_EDEVICE, std::string("job upload: ") + hipGetErrorString(e));
   if (!rc && ni) rc = accscan::launch_index_jobs(b->d_jobs, ni, vb_i, sb_i, b->stream);
   if (!rc && nj > ni) rc = accscan::launch_jobs(b->d_jobs + ni, nj - ni, vblocks, sblocks, b->stream);
-  for (int j = 0; j < nj && !rc; j++) {
-    const Query &q = s->queries[take[j]];
-    const ServedGraph &sg = s->graphs[q.gid];
-    const Slot &sl = s->slots[q.slot];
-    e = hipMemcpyAsync(sl.h_label, sl.buf.label, sizeof(int32_t) * sg.n, hipMemcpyDeviceToHost, b->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(sl.h_cid, sl.buf.cid, sizeof(int32_t) * sg.n, hipMemcpyDeviceToHost, b->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(sl.h_counters, sl.buf.counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                         b->stream);
-    if (e != hipSuccess) rc = acc_set_error(ACC_EDEVICE, std::string("result copy: ") + hipGetErrorString(e));
-  }
   if (!rc && (e = hipEventRecord(b->done, b->stream)) != hipSuccess)
     rc = acc_set_error(ACC_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
   if (rc) {
@@ -401,6 +420,7 @@ bool launch_batch(acc_server *s) {
   }
   b->busy = true;
   b->tickets = take;
+  s->cv_batch.notify_all();
   s->st.batches++;
   s->st.index_queries += static_cast<uint64_t>(ni);
   s->st.batched_queries += static_cast<uint64_t>(nj);
@@ -408,21 +428,17 @@ bool launch_batch(acc_server *s) {
   return true;
 }
 
+// Launch thread: admits graphs of pending queries and launches batches
+// while streams and result slots are free; completions come from the
+// per-stream waiter threads.
 void worker_main(acc_server *s) {
   (void)hipSetDevice(s->ctx->device);
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop) {
-    if (reap(s)) s->cv_done.notify_all();
-    bool launched = false;
-    while (admit_pending(s, lk)) launched = true;
-    while (launch_batch(s)) launched = true;
-    bool inflight = false;
-    for (const Batch &b : s->batches) inflight = inflight || b.busy;
-    if (launched) continue;
-    if (inflight)
-      s->cv_work.wait_for(lk, std::chrono::microseconds(20));
-    else
-      s->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+    bool progress = false;
+    while (admit_pending(s, lk)) progress = true;
+    while (launch_batch(s)) progress = true;
+    if (!progress) s->cv_work.wait_for(lk, std::chrono::milliseconds(50));
   }
 }
 
@@ -433,7 +449,10 @@ void destroy_server(acc_server *s) {
     s->stop = true;
   }
   s->cv_work.notify_all();
+  s->cv_batch.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  for (Batch &b : s->batches)
+    if (b.waiter.joinable()) b.waiter.join();
   (void)hipSetDevice(s->ctx->device);
   for (Batch &b : s->batches) {
     if (b.stream) (void)hipStreamSynchronize(b.stream);
@@ -487,6 +506,7 @@ int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **o
       return acc_set_error(ACC_EDEVICE, std::string("server setup: ") + hipGetErrorString(e));
     }
   }
+  for (Batch &b : s->batches) b.waiter = std::thread(waiter_main, s, &b);
   s->worker = std::thread(worker_main, s);
   *out = s;
   return ACC_OK;
@@ -591,7 +611,7 @@ int acc_server_poll(acc_server *s, uint64_t ticket, int *state) {
   return ACC_OK;
 }
 
-int acc_server_result(acc_server *s, uint64_t ticket, const int32_t **label, const int32_t **cluster_id,
+int acc_server_result(acc_server *s, uint64_t ticket, const int8_t **label, const int32_t **cluster_id,
                       uint32_t *n, int64_t *counts, double *latency_ms) {
   if (!s) return acc_set_error(ACC_EINVAL, "null server");
   std::lock_guard<std::mutex> lk(s->mu);

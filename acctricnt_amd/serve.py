@@ -92,16 +92,16 @@ class Server:
         return st.value
 
     def result(self, ticket: int, copy: bool = True) -> dict:
-        """labels, cluster ids (numpy views into pinned memory unless copy), the
-        four counters and the latency of a completed query."""
-        lab, cid = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+        """labels (int8), cluster ids (numpy views into pinned memory unless
+        copy), the four counters and the latency of a completed query."""
+        lab, cid = C.POINTER(C.c_int8)(), C.POINTER(C.c_int32)()
         n = C.c_uint32()
         counts = (C.c_int64 * 4)()
         lat = C.c_double()
         check_cn(self._lib.acc_server_result(self._h, ticket, C.byref(lab), C.byref(cid), C.byref(n), counts,
                                              C.byref(lat)))
         k = n.value
-        label = np.ctypeslib.as_array(lab, shape=(k,)) if k else np.zeros(0, np.int32)
+        label = np.ctypeslib.as_array(lab, shape=(k,)) if k else np.zeros(0, np.int8)
         cluster = np.ctypeslib.as_array(cid, shape=(k,)) if k else np.zeros(0, np.int32)
         if copy:
             label, cluster = label.copy(), cluster.copy()

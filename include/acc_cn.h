@@ -175,7 +175,8 @@ int acc_scan_pairs(acc_scan_ws *ws, int32_t *pairs, uint64_t cap, uint64_t *coun
  * a similarity index per graph (every vertex's slots sorted by a monotone
  * key of cn^2 / ((d_u+1)(d_v+1))), with which a query's core check is O(1)
  * per vertex and its clustering visits only similar slots; ties with eps
- * take the exact integer test, so results do not change.  Each query's labels (0 unclassified,
+ * take the exact integer test, so results do not change.  The last kernel
+ * of a query writes its results straight into pinned host memory.  Each query's labels (0 unclassified,
  * 1 core, 2 hub, as acc_scan_results) and cluster ids (-1 for non-cores) are
  * the ones acc_scan_run gives for the same graph and parameters alone; they
  * are copied to pinned host memory, readable with acc_server_result until
@@ -218,10 +219,11 @@ int acc_server_submit(acc_server *s, int32_t graph_id, int64_t eps_num, int64_t 
  * the query's own error code when it failed. */
 int acc_server_wait(acc_server *s, uint64_t ticket, int32_t timeout_ms);
 int acc_server_poll(acc_server *s, uint64_t ticket, int *state);
-/* Results of a completed query: pointers into pinned memory valid until
+/* Results of a completed query: labels (int8: 0 unclassified, 1 core, 2 hub)
+ * and cluster ids (-1 for non-cores) in pinned memory that stays valid until
  * acc_server_release; counts = cores, clusters, hubs, outliers; latency from
  * submission to completion. */
-int acc_server_result(acc_server *s, uint64_t ticket, const int32_t **label, const int32_t **cluster_id,
+int acc_server_result(acc_server *s, uint64_t ticket, const int8_t **label, const int32_t **cluster_id,
                       uint32_t *n, int64_t *counts, double *latency_ms);
 /* Frees a query's result slot; a pending query is cancelled. */
 int acc_server_release(acc_server *s, uint64_t ticket);

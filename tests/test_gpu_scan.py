@@ -122,3 +122,20 @@ def test_scan_full_size_rmat22(ctx):
     assert np.array_equal(res["cluster_id"][core], ref["cluster_id"][core])
     for k in ("core_num", "cluster_num", "hub_num", "outlier_num"):
         assert res[k] == ref[k], k
+
+
+def test_scan_repeated_low_eps_rmat18(ctx):
+    """Many unions (eps 0.1, mu 2 on R-MAT scale 18), repeated: every run must give
+    the oracle's labels and cluster ids (a root pointer overwritten by a racing
+    path-halving store used to leave a core outside its root now and then)."""
+    g = G.gen_rmat(18, 16, seed=1018)
+    ref = oracle.scan(g, oracle.cn_merge(g), "0.1", 2)
+    core = ref["label"] == 1
+    want_cid = np.where(core, ref["cluster_id"], -1)
+    with ctx.upload(g) as rg, rg.scan_workspace() as ws:
+        for rep in range(8):
+            ws.run("0.1", 2)
+            ws.synchronize()
+            res = ws.results()
+            assert np.array_equal(res["label"], ref["label"]), rep
+            assert np.array_equal(res["cluster_id"], want_cid), rep
