@@ -143,9 +143,14 @@ class ResidentGraph:
         return cnt, int(t.value)
 
     def triangle_total(self, path="auto") -> int:
+        """T; on the forward path a triangle-only pass that writes no per-slot counts."""
         t = C.c_uint64(0)
         check_cn(self._lib.acc_triangle_total(self._h, _path(path), C.byref(t)))
         return int(t.value)
+
+    def triangle_launch(self) -> None:
+        """Enqueue the triangle-only pass (asynchronous, for timing)."""
+        check_cn(self._lib.acc_triangle_launch(self._h))
 
     def timings(self) -> dict:
         ms = (C.c_float * 32)()

@@ -65,8 +65,9 @@ struct acc_graph {
   int32_t *d_adj = nullptr;
   int32_t *d_cnt = nullptr;
   uint32_t *d_rev = nullptr;            // reverse slot of every upper slot, ~0 elsewhere (built at upload)
-  unsigned long long *d_sum = nullptr;  // sum over upper slots of the raw counts (= 3T)
+  unsigned long long *d_sum = nullptr;  // [0]: sum over upper slots of the raw counts (= 3T); [1]: T alone
   int32_t *d_flags = nullptr;           // validation flags
+  unsigned long long *d_tri_parts = nullptr;  // partial sums of the triangle-only pass
   int32_t *d_src = nullptr;             // source vertex of every slot (built for SCAN)
   int32_t *d_old = nullptr;             // caller vertex ids for cluster ids (NULL: identity)
   // similarity index (built for serving): every vertex's slots sorted by a
@@ -206,6 +207,8 @@ int launch_index_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t 
 namespace accfwd {
 int launch_init(acc_graph *g);
 int launch_build_rev(acc_graph *g);
+// Triangle total only (forward path, no per-slot counts) into d_sum[1].
+int launch_triangles(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_count_big(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);

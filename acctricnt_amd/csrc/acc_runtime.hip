@@ -37,7 +37,8 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_old), static_cast<void *>(g->d_edge_units),
                   static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
                   static_cast<void *>(g->d_bmp_pool), static_cast<void *>(g->d_skey),
-                  static_cast<void *>(g->d_snbr), static_cast<void *>(g->d_sslot)})
+                  static_cast<void *>(g->d_snbr), static_cast<void *>(g->d_sslot),
+                  static_cast<void *>(g->d_tri_parts)})
     if (p) (void)hipFree(p);
   if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -247,7 +248,7 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
   if ((rc = dev_alloc(g, &g->d_adj, m + 64))) return bail(rc);
   if ((rc = dev_alloc(g, &g->d_cnt, m + 4))) return bail(rc);
   if ((rc = dev_alloc(g, &g->d_rev, m + 4))) return bail(rc);
-  if ((rc = dev_alloc(g, &g->d_sum, 1))) return bail(rc);
+  if ((rc = dev_alloc(g, &g->d_sum, 2))) return bail(rc);
   if ((rc = dev_alloc(g, &g->d_flags, 4))) return bail(rc);
   hipError_t e = hipMemcpyAsync(g->d_off, node_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
                                 hipMemcpyHostToDevice, ctx->stream);
@@ -619,9 +620,26 @@ int acc_cn_replay(acc_graph *g) {
 }
 
 int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles) {
-  if (!triangles) return acc_set_error(ACC_EINVAL, "null output");
-  ACC_TRY(acc_cn_compute(g, path, 0));
-  return acc_cn_download(g, nullptr, triangles);
+  if (!g || !triangles) return acc_set_error(ACC_EINVAL, "null argument");
+  if (path == ACC_PATH_AUTO) path = g->auto_path;
+  if (path != ACC_PATH_FORWARD) {
+    ACC_TRY(acc_cn_compute(g, path, 0));
+    return acc_cn_download(g, nullptr, triangles);
+  }
+  // forward lists without per-slot outputs: every triangle counted once at its middle vertex
+  DeviceGuard guard(g->ctx->device);
+  ACC_TRY(accfwd::launch_triangles(g));
+  unsigned long long t = 0;
+  ACC_HIP_TRY(hipMemcpyAsync(&t, g->d_sum + 1, sizeof(t), hipMemcpyDeviceToHost, g->ctx->stream));
+  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  *triangles = t;
+  return ACC_OK;
+}
+
+int acc_triangle_launch(acc_graph *g) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  DeviceGuard guard(g->ctx->device);
+  return accfwd::launch_triangles(g);
 }
 
 }  // extern "C"

@@ -190,6 +190,15 @@ __device__ __forceinline__ void count_round(int32_t *reg, int k, bool hit, int l
   }
 }
 
+// Partial triangle totals of the triangle-only pass: kTriParts counters one
+// 128-byte line apart, workgroup w adding into counter w mod kTriParts, so
+// the atomics of many small units do not queue on one address.
+constexpr int kTriParts = 256;
+constexpr int kTriStride = 16;  // unsigned long longs per 128-byte line
+__device__ __forceinline__ unsigned long long *tri_part(unsigned long long *parts) {
+  return parts + (blockIdx.x % kTriParts) * kTriStride;
+}
+
 // Wave-flattened probing of up to 64 upper slots (b, c_k), one per lane k:
 // lane k holds cb = start of N(c_k) and P = the exclusive prefix of the
 // probe lengths L_k = |N(c_k) ∩ [0, b)| (lanes without a probe hold P = T,
@@ -197,10 +206,13 @@ __device__ __forceinline__ void count_round(int32_t *reg, int k, bool hit, int l
 // k(e) = max{k : P_k <= e} (a 6-step search over the lanes) and is
 // N(c_k)[e - P_k]; every lane takes one element per round, U rounds per
 // iteration, so short lists leave no lane idle and no load passes b.
-template <int U, class Set>
-__device__ __forceinline__ void wave_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
-                                                int32_t *ctr, uint32_t ctr_cap, uint32_t sb, uint32_t P,
-                                                uint32_t cb, uint32_t T, int32_t *reg, int lane) {
+// With kSlots false only the number of triangles found is kept (returned
+// per lane); nothing is written.
+template <int U, bool kSlots, class Set>
+__device__ __forceinline__ uint32_t wave_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
+                                                    int32_t *ctr, uint32_t ctr_cap, uint32_t sb, uint32_t P,
+                                                    uint32_t cb, uint32_t T, int32_t *reg, int lane) {
+  uint32_t found = 0;
   for (uint32_t e0 = 0; e0 < T; e0 += 64 * U) {
     int32_t a[U];
     uint32_t idx[U];
@@ -222,13 +234,17 @@ __device__ __forceinline__ void wave_probe_flat(const int32_t *__restrict__ adj,
       if (a[q] >= 0) {
         const int pos = set.find(a[q]);
         if (pos >= 0) {
-          count_triangle(cnt, ctr, ctr_cap, sb, idx[q], pos);
+          if (kSlots) count_triangle(cnt, ctr, ctr_cap, sb, idx[q], pos);
           hit = true;
         }
       }
-      count_round(reg, kq[q], hit, lane);
+      if (kSlots)
+        count_round(reg, kq[q], hit, lane);
+      else
+        found += hit;
     }
   }
+  return found;
 }
 
 constexpr int kWT = 1 << accfwd::kWaveTableBits;
@@ -243,12 +259,13 @@ constexpr size_t kCountLdsBytes = kBlockLdsBytes > kWaveLdsBytes ? kBlockLdsByte
 // w, w + kWaves, ... of 64 * U consecutive elements, so the chunk's work is
 // split evenly over the waves whatever the list lengths; k(e) is a 9-step
 // search of s_P.  Round indices are wave-uniform.
-template <int U, class Set>
-__device__ __forceinline__ void block_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
-                                                 int32_t *ctr, uint32_t sb, const uint32_t *s_P,
-                                                 const uint32_t *s_cb, uint32_t T, int32_t *s_reg, int wave,
-                                                 int lane) {
+template <int U, bool kSlots, class Set>
+__device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
+                                                     int32_t *ctr, uint32_t sb, const uint32_t *s_P,
+                                                     const uint32_t *s_cb, uint32_t T, int32_t *s_reg, int wave,
+                                                     int lane) {
   static_assert(kChunk == 512, "the search below covers 512 entries");
+  uint32_t found = 0;
   for (uint32_t e0 = static_cast<uint32_t>(wave) * 64 * U; e0 < T; e0 += kWaves * 64 * U) {
     int32_t a[U];
     uint32_t idx[U];
@@ -270,23 +287,28 @@ __device__ __forceinline__ void block_probe_flat(const int32_t *__restrict__ adj
       if (a[q] >= 0) {
         const int pos = set.find(a[q]);
         if (pos >= 0) {
-          count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
+          if (kSlots) count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
           hit = true;
         }
       }
-      count_round(s_reg, kq[q], hit, lane);
+      if (kSlots)
+        count_round(s_reg, kq[q], hit, lane);
+      else
+        found += hit;
     }
   }
+  return found;
 }
 
 // Workgroup unit: one chunk of at most kChunk upper slots of a source b.
 // N<(b) is an LDS bitmap when b - min N(b) <= kBitmapIds (|N<(b)| < 65536),
 // otherwise an LDS hash table (|N<(b)| <= kLdsCap).  Both layouts share the
 // counter and probe-metadata regions of smem.
-template <int U>
+// kSlots false: count the unit's triangles into *total instead of writing slots.
+template <int U, bool kSlots>
 __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                                const uint32_t *__restrict__ rev, const FwdUnit u, int32_t *cnt,
-                                               unsigned char *smem) {
+                                               unsigned long long *total, unsigned char *smem) {
   int32_t *keys = reinterpret_cast<int32_t *>(smem);  // hash keys, or bitmap words
   int32_t *ctr = keys + kBT;
   uint32_t *s_cb = reinterpret_cast<uint32_t *>(ctr + kLdsCap);
@@ -310,7 +332,8 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   const uint32_t nclear = bitmap ? (span + 31) / 32 : mask + 1;
   const int32_t fill = bitmap ? 0 : -1;
   for (uint32_t i = tid; i < nclear; i += kWG) keys[i] = fill;
-  for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
+  if (kSlots)
+    for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
   // probe lengths L_t = |N(c_t) ∩ [0, b)| = rev - start of N(c_t), and their prefix
   uint32_t len = 0;
   if (static_cast<uint32_t>(tid) < nup) {
@@ -344,12 +367,18 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   }
   __syncthreads();
   T = __builtin_amdgcn_readfirstlane(T);
+  uint32_t found;
   if (bitmap) {
     const BitSet set{reinterpret_cast<const uint32_t *>(keys), kpos, lo};
-    block_probe_flat<U>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
+    found = block_probe_flat<U, kSlots>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
   } else {
     const HashSet set{keys, kpos, bits, mask};
-    block_probe_flat<U>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
+    found = block_probe_flat<U, kSlots>(adj, cnt, set, ctr, sb, s_P, s_cb, T, s_reg, wave, lane);
+  }
+  if (!kSlots) {
+    const unsigned long long w = wave_sum_u64(found);
+    if (lane == 0 && w) atomicAdd(tri_part(total), w);
+    return;
   }
   __syncthreads();
   if (static_cast<uint32_t>(tid) < nup) cnt[u.j0 + tid] = s_reg[tid];
@@ -361,10 +390,10 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
 
 // Wave unit: a source with at most kWaveSlots upper slots and
 // |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.
-template <int U>
+template <int U, bool kSlots>
 __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                               const uint32_t *__restrict__ rev, const FwdUnit u, int32_t *cnt,
-                                              unsigned char *smem, int wave) {
+                                              unsigned long long *total, unsigned char *smem, int wave) {
   int32_t *base = reinterpret_cast<int32_t *>(smem);
   int32_t *keys = base + wave * kWT;
   int32_t *ctr = base + kWaves * kWT + wave * accfwd::kWaveCap;
@@ -388,13 +417,20 @@ __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, 
   const uint32_t P = wave_excl_scan(len, lane);
   const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
   for (uint32_t i = lane; i <= mask; i += 64) keys[i] = -1;
-  for (uint32_t i = lane; i < nl; i += 64) ctr[i] = 0;
-  reg[lane] = 0;
+  if (kSlots) {
+    for (uint32_t i = lane; i < nl; i += 64) ctr[i] = 0;
+    reg[lane] = 0;
+  }
   wave_lds_sync();
   lds_insert(keys, kpos, bits, mask, adj, sb, nl, lane, 64);
   wave_lds_sync();
   const HashSet set{keys, kpos, bits, mask};
-  wave_probe_flat<U>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, P, cb, T, reg, lane);
+  const uint32_t found = wave_probe_flat<U, kSlots>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, P, cb, T, reg, lane);
+  if (!kSlots) {
+    const unsigned long long w = wave_sum_u64(found);
+    if (lane == 0 && w) atomicAdd(tri_part(total), w);
+    return;
+  }
   wave_lds_sync();
   if (static_cast<uint32_t>(lane) < nup) cnt[u.j0 + lane] = reg[lane];
   for (uint32_t i = lane; i < nl; i += 64) {
@@ -408,22 +444,23 @@ __device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, 
 // w, w + gridDim.x, ... (one task per workgroup when the grid covers them, a
 // persistent grid otherwise).  The task index is workgroup-uniform, so the
 // barrier that separates consecutive tasks' use of LDS is reached by all.
-template <int U>
+// kSlots false: the triangle total only, accumulated into the partial sums at total.
+template <int U, bool kSlots>
 __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
                                                    const int32_t *__restrict__ adj,
                                                    const uint32_t *__restrict__ rev,
                                                    const FwdUnit *__restrict__ block_units, uint32_t n_block,
                                                    const FwdUnit *__restrict__ wave_units, uint32_t n_wave,
-                                                   int32_t *cnt) {
+                                                   int32_t *cnt, unsigned long long *total) {
   __shared__ __align__(16) unsigned char smem[kCountLdsBytes];
   const uint32_t tasks = n_block + (n_wave + kWaves - 1) / kWaves;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   for (uint32_t t = blockIdx.x; t < tasks; t += gridDim.x) {
     if (t < n_block) {
-      fwd_block_unit<U>(off, adj, rev, block_units[t], cnt, smem);
+      fwd_block_unit<U, kSlots>(off, adj, rev, block_units[t], cnt, total, smem);
     } else {
       const uint32_t ui = (t - n_block) * kWaves + wave;
-      if (ui < n_wave) fwd_wave_unit<U>(off, adj, rev, wave_units[ui], cnt, smem, wave);
+      if (ui < n_wave) fwd_wave_unit<U, kSlots>(off, adj, rev, wave_units[ui], cnt, total, smem, wave);
     }
     __syncthreads();
   }
@@ -432,10 +469,12 @@ __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ 
 // Sources whose N<(b) exceeds the workgroup table: membership by binary
 // search of N<(b) in global memory, (b,a) counted by direct atomics.  One
 // wave per upper slot; the probe stops at the reverse slot.
+template <bool kSlots>
 __global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__restrict__ off,
                                                             const int32_t *__restrict__ adj,
                                                             const uint32_t *__restrict__ rev,
-                                                            const FwdUnit *__restrict__ units, int32_t *cnt) {
+                                                            const FwdUnit *__restrict__ units, int32_t *cnt,
+                                                            unsigned long long *total) {
   const FwdUnit u = units[blockIdx.x];
   const int32_t b = u.b;
   const uint32_t sb = off[b];
@@ -462,13 +501,20 @@ __global__ __launch_bounds__(accfwd::kBigWG) void k_fwd_big(const uint32_t *__re
         }
         if (lo < nl && adj[sb + lo] == a) {
           reg++;
-          atomicAdd(&cnt[idx], 1);
-          atomicAdd(&cnt[sb + lo], 1);
+          if (kSlots) {
+            atomicAdd(&cnt[idx], 1);
+            atomicAdd(&cnt[sb + lo], 1);
+          }
         }
       }
     }
     reg = wave_sum_i32(reg);
-    if (lane == 0) cnt[j] = reg;
+    if (lane == 0) {
+      if (kSlots)
+        cnt[j] = reg;
+      else if (reg)
+        atomicAdd(tri_part(total), static_cast<unsigned long long>(reg));
+    }
   }
 }
 
@@ -494,6 +540,23 @@ __global__ __launch_bounds__(256) void k_combine(int32_t *__restrict__ cnt, cons
   if (threadIdx.x == 0) {
     const unsigned long long t = part[0] + part[1] + part[2] + part[3];
     if (t) atomicAdd(sum, t);
+  }
+}
+
+__global__ __launch_bounds__(kTriParts) void k_tri_zero(unsigned long long *__restrict__ parts) {
+  parts[threadIdx.x * kTriStride] = 0;
+}
+
+__global__ __launch_bounds__(kTriParts) void k_tri_sum(const unsigned long long *__restrict__ parts,
+                                                       unsigned long long *__restrict__ total) {
+  __shared__ unsigned long long w[kTriParts / 64];
+  const unsigned long long v = wave_sum_u64(parts[threadIdx.x * kTriStride]);
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kTriParts / 64; i++) t += w[i];
+    *total = t;
   }
 }
 
@@ -601,7 +664,8 @@ int launch_build_rev(acc_graph *g) {
   return ACC_OK;
 }
 
-int launch_count(acc_graph *g) {
+template <bool kSlots>
+int launch_count_t(acc_graph *g, unsigned long long *total) {
   hipStream_t st = g->ctx->stream;
   const uint64_t tasks = g->n_fwd_block + (g->n_fwd_wave + kWaves - 1) / kWaves;
   const uint64_t cap = static_cast<uint64_t>(g->persist) * static_cast<uint64_t>(g->ctx->num_cus);
@@ -609,16 +673,16 @@ int launch_count(acc_graph *g) {
   if (grid) {
     switch (g->fwd_unroll) {
       case 1:
-        hipLaunchKernelGGL(k_fwd_count<1>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
-                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
+        hipLaunchKernelGGL((k_fwd_count<1, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
       case 4:
-        hipLaunchKernelGGL(k_fwd_count<4>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
-                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
+        hipLaunchKernelGGL((k_fwd_count<4, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
       default:
-        hipLaunchKernelGGL(k_fwd_count<2>, dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
-                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt);
+        hipLaunchKernelGGL((k_fwd_count<2, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+                           g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
     }
     ACC_HIP_TRY(hipGetLastError());
@@ -626,12 +690,36 @@ int launch_count(acc_graph *g) {
   return ACC_OK;
 }
 
-int launch_count_big(acc_graph *g) {
+template <bool kSlots>
+int launch_big_t(acc_graph *g, unsigned long long *total) {
   if (g->n_fwd_big) {
-    hipLaunchKernelGGL(k_fwd_big, dim3(g->n_fwd_big), dim3(kBigWG), 0, g->ctx->stream, g->d_off, g->d_adj,
-                       g->d_rev, g->d_fwd_big, g->d_cnt);
+    hipLaunchKernelGGL(k_fwd_big<kSlots>, dim3(g->n_fwd_big), dim3(kBigWG), 0, g->ctx->stream, g->d_off, g->d_adj,
+                       g->d_rev, g->d_fwd_big, g->d_cnt, total);
     ACC_HIP_TRY(hipGetLastError());
   }
+  return ACC_OK;
+}
+
+int launch_count(acc_graph *g) { return launch_count_t<true>(g, nullptr); }
+
+int launch_count_big(acc_graph *g) { return launch_big_t<true>(g, nullptr); }
+
+int launch_triangles(acc_graph *g) {
+  if (!g->d_tri_parts) {
+    if (hipMalloc(reinterpret_cast<void **>(&g->d_tri_parts), sizeof(unsigned long long) * kTriParts * kTriStride) !=
+        hipSuccess) {
+      g->d_tri_parts = nullptr;
+      return acc_set_error(ACC_ENOMEM, "hipMalloc of the triangle partial sums failed");
+    }
+    g->device_bytes += sizeof(unsigned long long) * kTriParts * kTriStride;
+  }
+  hipStream_t st = g->ctx->stream;
+  hipLaunchKernelGGL(k_tri_zero, dim3(1), dim3(kTriParts), 0, st, g->d_tri_parts);
+  ACC_HIP_TRY(hipGetLastError());
+  ACC_TRY(launch_count_t<false>(g, g->d_tri_parts));
+  ACC_TRY(launch_big_t<false>(g, g->d_tri_parts));
+  hipLaunchKernelGGL(k_tri_sum, dim3(1), dim3(kTriParts), 0, st, g->d_tri_parts, g->d_sum + 1);
+  ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
 

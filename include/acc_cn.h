@@ -126,9 +126,14 @@ int acc_cn_capture(acc_graph *g, int path, uint32_t flags);
 int acc_cn_replay(acc_graph *g);
 
 /* Triangle count (SCAN_XP::TriCnt / TriCntSortedArray,
- * scan-query/scan_xp_triangle_cnt.cpp:778-957): runs the counting path and
- * returns T. */
+ * scan-query/scan_xp_triangle_cnt.cpp:778-957).  On the forward path (and
+ * ACC_PATH_AUTO resolving to it) a dedicated pass counts every triangle once
+ * at its middle vertex and writes no per-slot counts; the bitmap and merge
+ * paths run their full count and return its T. */
 int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles);
+/* Enqueues that triangle-only pass on the context stream without waiting
+ * (for timing; the total is what acc_triangle_total returns). */
+int acc_triangle_launch(acc_graph *g);
 
 /* SCAN queries ---------------------------------------------------------- */
 /* Structural clustering on the resident counts, replacing the reference's

@@ -221,3 +221,74 @@ def test_full_size_rmat22_properties(ctx):
     mask = ref >= 0
     assert mask.sum() > 0
     assert np.array_equal(cnt[mask], ref[mask])
+
+
+@pytest.mark.parametrize("path", PATHS + ["auto"])
+def test_triangle_total(ctx, path):
+    """T from acc_triangle_total (the triangle-only pass on the forward path)
+    against the oracle on seeded graphs, cliques and sources of every bin."""
+    cases = [G.gen_rmat(14, 16, seed=71), G.gen_er(4000, 20, seed=72), G.gen_chung_lu(1 << 14, 2.1, 24, 4096, seed=73),
+             wide_span_graph(150_000, 100, 900, 700, seed=74), wide_span_graph(150_000, 1000, 2000, 300, seed=75)]
+    s, d = np.triu_indices(2600, 1)
+    cases.append(G.from_edges(2600, s, d))
+    cases.append(G.from_csr(np.array([0, 0], np.uint32), np.zeros(0, np.int32)))
+    for g in cases:
+        with ctx.upload(g) as rg:
+            assert rg.triangle_total(path) == oracle.triangle_count(g), g.name
+
+
+def test_triangle_total_leaves_counts(ctx):
+    """The triangle-only pass writes no per-slot counts: counts and their total
+    from an earlier full pass stay as they were."""
+    g = G.gen_rmat(15, 16, seed=76)
+    with ctx.upload(g) as rg:
+        cnt, t = rg.count_all_edges("forward")
+        for name, value in (("chunk", 128), ("wave_slots", 0), ("persist", 2), ("fwd_unroll", 1)):
+            rg.set_param(name, value)
+            assert rg.triangle_total("forward") == t, (name, value)
+        cnt2, t2 = rg.download()
+    assert np.array_equal(cnt, cnt2) and t2 == t
+
+
+def test_triangle_total_rmat22(ctx):
+    g = G.gen_rmat(22, 16, seed=1)
+    with ctx.upload(g) as rg:
+        assert rg.triangle_total("forward") == oracle.triangle_count(g)
+
+
+def test_full_size_chunglu23(ctx):
+    """Chung-Lu, exponent 2.1, n = 2^23, average degree 32, expected degrees
+    capped at 2^20: symmetry, sum = 6T, T equal to the oracle's, bit-exact
+    counts on 100k sampled u<v slots and their reverse slots, and SCAN at
+    eps 0.3 and 0.6, mu 5, equal to the oracle's SCAN over the same counts."""
+    g = G.gen_chung_lu(1 << 23, 2.1, 32, 1 << 20, seed=1)
+    print(f"[chunglu23] n={g.n} m={g.m} max_deg={int(g.degrees().max())}", flush=True)
+    with ctx.upload(g) as rg:
+        cnt, t = rg.count_all_edges("forward")
+        assert rg.triangle_total("forward") == t
+        scans = {}
+        with rg.scan_workspace() as ws:
+            for eps in ("0.3", "0.6"):
+                ws.run(eps, 5)
+                ws.synchronize()
+                scans[eps] = ws.results()
+    rev = g.reverse_slots()
+    assert np.array_equal(cnt, cnt[rev])
+    assert int((cnt.astype(np.int64) - 2).sum()) == 6 * t
+    assert t == oracle.triangle_count(g)
+    rng = np.random.default_rng(1)
+    src = g.sources()
+    up = np.nonzero(src < g.edge_dst)[0]
+    sel = rng.choice(up, 100_000, replace=False)
+    ref = np.full(g.m, -1, np.int32)
+    oracle.cn_merge_slots(g, ref, sel, src[sel])
+    mask = ref >= 0
+    assert mask.sum() == 2 * sel.size
+    assert np.array_equal(cnt[mask], ref[mask])
+    for eps, res in scans.items():
+        want = oracle.scan(g, cnt, eps, 5)
+        assert np.array_equal(res["label"], want["label"]), eps
+        core = want["label"] == 1
+        assert np.array_equal(res["cluster_id"][core], want["cluster_id"][core]), eps
+        for k in ("core_num", "cluster_num", "hub_num", "outlier_num"):
+            assert res[k] == want[k], (eps, k)
