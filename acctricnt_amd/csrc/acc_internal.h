@@ -64,6 +64,7 @@ struct acc_graph {
   uint32_t *d_off = nullptr;
   int32_t *d_adj = nullptr;
   int32_t *d_cnt = nullptr;
+  uint32_t *d_hs = nullptr;             // first upper slot of every vertex
   uint32_t *d_rev = nullptr;            // reverse slot of every upper slot, ~0 elsewhere (built at upload)
   unsigned long long *d_sum = nullptr;  // [0]: sum over upper slots of the raw counts (= 3T); [1]: T alone
   int32_t *d_flags = nullptr;           // validation flags
@@ -77,7 +78,7 @@ struct acc_graph {
   uint32_t *d_sslot = nullptr; // original slot of every sorted position
   bool cnt_plus2 = true;                // counts stored as 2 + |N(u) ∩ N(v)|
   // forward plan, binned by degree skew
-  FwdUnit *d_fwd_wave = nullptr;  // few upper slots, small N<(b): one wave per source
+  FwdUnit *d_fwd_wave = nullptr;  // wave packs of small sources: {b0, b1, upper slots, lower entries}
   uint32_t n_fwd_wave = 0;
   FwdUnit *d_fwd_block = nullptr;  // one workgroup per chunk of upper slots, N<(b) in LDS
   uint32_t n_fwd_block = 0;

@@ -38,7 +38,7 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
                   static_cast<void *>(g->d_bmp_pool), static_cast<void *>(g->d_skey),
                   static_cast<void *>(g->d_snbr), static_cast<void *>(g->d_sslot),
-                  static_cast<void *>(g->d_tri_parts)})
+                  static_cast<void *>(g->d_tri_parts), static_cast<void *>(g->d_hs)})
     if (p) (void)hipFree(p);
   if (g->graph_exec) (void)hipGraphExecDestroy(g->graph_exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -50,22 +50,63 @@ void free_graph(acc_graph *g) {
 }
 
 // Forward plan, binned by degree skew: a source b with upper slots (neighbors
-// > b) becomes one wave unit when it has at most wave_slots upper slots and
-// |N<(b)| <= wave_low, otherwise workgroup units of at most `chunk` upper
-// slots (N<(b) as an LDS bitmap when b - lo[b] <= kBitmapIds, else an LDS
-// hash table); sources whose N<(b) fits neither go to the global-memory
-// variant.  hs[b] is the first upper slot of b, lo[b] its smallest neighbor.
+// > b) is a small source when it has at most wave_slots upper slots and
+// |N<(b)| <= wave_low; runs of consecutive small sources form wave packs
+// (FwdUnit {b0, b1, upper slots, lower entries}: at most 64 sources, 64
+// upper slots, 256 lower entries and 512 hash-table entries, each source's
+// table being the smallest power of two >= 2 |N<(b)|, at least 16).  Other
+// sources become workgroup units of at most `chunk` upper slots (N<(b) as an
+// LDS bitmap when b - lo[b] <= kBitmapIds, else an LDS hash table); sources
+// whose N<(b) fits neither go to the global-memory variant.  hs[b] is the
+// first upper slot of b, lo[b] its smallest neighbor.
 void build_forward_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, const int32_t *lo, uint32_t wave_slots,
-                        uint32_t wave_low, uint32_t chunk, std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block,
-                        std::vector<FwdUnit> &big) {
+                        uint32_t wave_low, uint32_t chunk, uint32_t wave_slots_on_device, std::vector<FwdUnit> &wave,
+                        std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
+  // Sources per pack: just enough packs to give every wave slot of the
+  // device one (a small graph keeps one source per wave), at most 64.
+  uint64_t n_small = 0;
+  for (uint32_t b = 0; b < n; b++) {
+    const uint32_t nl = hs[b] - off[b], nup = off[b + 1] - hs[b];
+    n_small += nup && nup <= wave_slots && nl <= wave_low;
+  }
+  const uint64_t per = (n_small + wave_slots_on_device - 1) / std::max<uint32_t>(wave_slots_on_device, 1);
+  const uint32_t max_sources = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, per)));
+  auto table = [](uint32_t nl) -> uint32_t {
+    if (!nl) return 0;
+    uint32_t t = 16;
+    while (t < 2 * nl) t *= 2;
+    return t;
+  };
+  FwdUnit pack{0, 0, 0, 0};
+  uint32_t pack_tbl = 0, pack_sources = 0;
+  bool open = false;
+  auto close = [&]() {
+    if (open) wave.push_back(pack);
+    open = false;
+  };
   for (uint32_t b = 0; b < n; b++) {
     const uint32_t nl = hs[b] - off[b];
     const uint32_t nup = off[b + 1] - hs[b];
     if (nup == 0) continue;
     if (nup <= wave_slots && nl <= wave_low) {
-      wave.push_back(FwdUnit{static_cast<int32_t>(b), hs[b], off[b + 1], nl});
+      const uint32_t t = table(nl);
+      if (open && (b + 1 - static_cast<uint32_t>(pack.b) > 64 || pack_sources == max_sources || pack.j1 + nup > 64 ||
+                   pack.nlow + nl > accfwd::kWaveCap || pack_tbl + t > (1u << accfwd::kWaveTableBits)))
+        close();
+      if (!open) {
+        pack = FwdUnit{static_cast<int32_t>(b), b + 1, 0, 0};
+        pack_tbl = 0;
+        pack_sources = 0;
+        open = true;
+      }
+      pack_sources++;
+      pack.j0 = b + 1;
+      pack.j1 += nup;
+      pack.nlow += nl;
+      pack_tbl += t;
       continue;
     }
+    close();
     for (uint32_t j0 = hs[b]; j0 < off[b + 1]; j0 += chunk) {
       FwdUnit u{static_cast<int32_t>(b), j0, std::min(j0 + chunk, off[b + 1]), nl};
       const uint32_t span = b - static_cast<uint32_t>(lo[b]);
@@ -73,6 +114,7 @@ void build_forward_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, con
       (lds ? block : big).push_back(u);
     }
   }
+  close();
 }
 
 // Per-edge plan: chunks of at most `chunk` upper slots of every source (merge
@@ -112,13 +154,19 @@ void drop_capture(acc_graph *g) {
 int upload_plans(acc_graph *g) {
   drop_capture(g);
   std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
-  build_forward_plan(g->n, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots, g->wave_low, g->chunk, wave, block, big);
+  const uint32_t device_waves = static_cast<uint32_t>(std::max(1, g->ctx->num_cus)) * 32u;
+  build_forward_plan(g->n, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots, g->wave_low, g->chunk,
+                     device_waves, wave, block, big);
   build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
   struct PlanList {
     std::vector<FwdUnit> *v;
     FwdUnit **d;
     uint32_t *count;
   };
+  if (!g->d_hs) {
+    ACC_TRY(dev_alloc(g, &g->d_hs, static_cast<uint64_t>(g->n) + 1));
+    ACC_HIP_TRY(hipMemcpyAsync(g->d_hs, g->h_hs.data(), sizeof(uint32_t) * g->n, hipMemcpyHostToDevice, g->ctx->stream));
+  }
   PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
                       {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
                       {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};

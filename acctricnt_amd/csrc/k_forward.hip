@@ -199,54 +199,6 @@ __device__ __forceinline__ unsigned long long *tri_part(unsigned long long *part
   return parts + (blockIdx.x % kTriParts) * kTriStride;
 }
 
-// Wave-flattened probing of up to 64 upper slots (b, c_k), one per lane k:
-// lane k holds cb = start of N(c_k) and P = the exclusive prefix of the
-// probe lengths L_k = |N(c_k) ∩ [0, b)| (lanes without a probe hold P = T,
-// the total).  Element e of the flattened range [0, T) belongs to probe
-// k(e) = max{k : P_k <= e} (a 6-step search over the lanes) and is
-// N(c_k)[e - P_k]; every lane takes one element per round, U rounds per
-// iteration, so short lists leave no lane idle and no load passes b.
-// With kSlots false only the number of triangles found is kept (returned
-// per lane); nothing is written.
-template <int U, bool kSlots, class Set>
-__device__ __forceinline__ uint32_t wave_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
-                                                    int32_t *ctr, uint32_t ctr_cap, uint32_t sb, uint32_t P,
-                                                    uint32_t cb, uint32_t T, int32_t *reg, int lane) {
-  uint32_t found = 0;
-  for (uint32_t e0 = 0; e0 < T; e0 += 64 * U) {
-    int32_t a[U];
-    uint32_t idx[U];
-    int kq[U];
-#pragma unroll
-    for (int q = 0; q < U; q++) {
-      const uint32_t e = e0 + 64 * q + lane;
-      int k = 0;
-#pragma unroll
-      for (int st = 32; st; st >>= 1)
-        if (static_cast<uint32_t>(__shfl(P, k + st)) <= e) k += st;
-      kq[q] = k;
-      idx[q] = static_cast<uint32_t>(__shfl(cb, k)) + (e - static_cast<uint32_t>(__shfl(P, k)));
-      a[q] = e < T ? adj[idx[q]] : -1;
-    }
-#pragma unroll
-    for (int q = 0; q < U; q++) {
-      bool hit = false;
-      if (a[q] >= 0) {
-        const int pos = set.find(a[q]);
-        if (pos >= 0) {
-          if (kSlots) count_triangle(cnt, ctr, ctr_cap, sb, idx[q], pos);
-          hit = true;
-        }
-      }
-      if (kSlots)
-        count_round(reg, kq[q], hit, lane);
-      else
-        found += hit;
-    }
-  }
-  return found;
-}
-
 constexpr int kWT = 1 << accfwd::kWaveTableBits;
 constexpr int kBT = 1 << kTableBits;
 constexpr size_t kBlockLdsBytes = 4 * kBT + 2 * kBT + 4 * kLdsCap + 12 * kChunk + 4 * kWaves;
@@ -388,65 +340,178 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   }
 }
 
-// Wave unit: a source with at most kWaveSlots upper slots and
-// |N<(b)| <= kWaveCap, processed by one wave with a wave-private table.
+// Largest k in [0, 64) with X_k <= e, for X nondecreasing across the lanes
+// and X_0 <= e (a 6-step search over the lanes; every lane must call it).
+__device__ __forceinline__ int lane_search(uint32_t X, uint32_t e) {
+  int k = 0;
+#pragma unroll
+  for (int st = 32; st; st >>= 1)
+    if (static_cast<uint32_t>(__shfl(X, k + st)) <= e) k += st;
+  return k;
+}
+
+// Hash table size of one source of a wave pack: the smallest power of two
+// >= 2 |N<(b)|, at least 16 entries.
+__device__ __forceinline__ int pack_bits(uint32_t nl) {
+  int bits = 4;
+  while ((1u << bits) < 2 * nl) bits++;
+  return bits;
+}
+
+// Wave pack: the consecutive sources [b0, b1) (at most 64), each with at
+// most wave_slots upper slots and |N<(b)| <= wave_low, together at most 64
+// upper slots, 256 lower entries and 512 hash entries (the plan guarantees
+// these bounds).  One wave builds one LDS hash table per source, then
+// flattens the probes of all the pack's upper slots into one element range:
+// element e belongs to probe i = k(e), whose source's table decides it.
+// Packing several small sources into one wave shares the latency of the
+// metadata loads and the fixed work of a unit between them.
 template <int U, bool kSlots>
-__device__ __forceinline__ void fwd_wave_unit(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                              const uint32_t *__restrict__ rev, const FwdUnit u, int32_t *cnt,
-                                              unsigned long long *total, unsigned char *smem, int wave) {
+__device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, const uint32_t *__restrict__ hs,
+                                              const int32_t *__restrict__ adj, const uint32_t *__restrict__ rev,
+                                              const FwdUnit u, int32_t *cnt, unsigned long long *total,
+                                              unsigned char *smem, int wave) {
   int32_t *base = reinterpret_cast<int32_t *>(smem);
   int32_t *keys = base + wave * kWT;
   int32_t *ctr = base + kWaves * kWT + wave * accfwd::kWaveCap;
   int32_t *reg = base + kWaves * (kWT + accfwd::kWaveCap) + wave * 64;
   uint16_t *kpos = reinterpret_cast<uint16_t *>(base + kWaves * (kWT + accfwd::kWaveCap + 64)) + wave * kWT;
   const int lane = threadIdx.x & 63;
-  const int32_t b = u.b;
-  const uint32_t sb = off[b];
-  const uint32_t nl = u.nlow;
-  const uint32_t nup = u.j1 - u.j0;
-  const int bits = table_bits_for(nl);
-  const uint32_t mask = (1u << bits) - 1;
-  // one upper slot per lane: start of N(c) and probe length |N(c) ∩ [0, b)|
-  uint32_t cb = 0, len = 0;
-  if (static_cast<uint32_t>(lane) < nup) {
-    const int32_t c = adj[u.j0 + lane];
-    cb = off[c];
-    const uint32_t p = rev[u.j0 + lane];
-    len = p != ~0u ? p - cb : 0;
+  const int32_t b0 = u.b;
+  const uint32_t nv = u.j0 - static_cast<uint32_t>(b0);
+  const uint32_t nup_tot = u.j1;
+  const uint32_t nl_tot = u.nlow;
+  // one source per lane: lower slots [o0, h), upper slots [h, o1)
+  uint32_t o0 = 0, h = 0, nupv = 0, nlv = 0, tsz = 0;
+  int bits = 0;
+  if (static_cast<uint32_t>(lane) < nv) {
+    const int32_t v = b0 + lane;
+    o0 = off[v];
+    h = hs[v];
+    nupv = off[v + 1] - h;
+    if (nupv) {
+      nlv = h - o0;
+      if (nlv) {
+        bits = pack_bits(nlv);
+        tsz = 1u << bits;
+      }
+    }
   }
-  const uint32_t P = wave_excl_scan(len, lane);
-  const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
-  for (uint32_t i = lane; i <= mask; i += 64) keys[i] = -1;
+  const uint32_t Q = wave_excl_scan(nupv, lane);   // first probe of each source
+  const uint32_t NL = wave_excl_scan(nlv, lane);   // first counter of each source
+  const uint32_t TB = wave_excl_scan(tsz, lane);   // first table entry of each source
+  const uint32_t tbl_tot = __builtin_amdgcn_readfirstlane(__shfl(TB + tsz, 63));
+  // one probe per lane: its source, upper slot, N(c) start and length
+  const int pk = lane_search(Q, static_cast<uint32_t>(lane));
+  const uint32_t j = static_cast<uint32_t>(__shfl(h, pk)) + (static_cast<uint32_t>(lane) - __shfl(Q, pk));
+  const uint32_t p_tb = __shfl(TB, pk);
+  const int p_bits = __shfl(bits, pk);  // 0: the probe's source has an empty N<(b) and no table
+  const uint32_t p_nl = __shfl(NL, pk);
+  const bool probe = static_cast<uint32_t>(lane) < nup_tot;
+  uint32_t cb = 0, len = 0;
+  if (probe) {
+    const int32_t c = adj[j];
+    const uint32_t p = rev[j];
+    cb = off[c];
+    // a source with an empty N<(b) is the middle of no triangle: nothing to probe
+    len = p != ~0u && p_bits > 0 ? p - cb : 0;
+  }
+  // the pack's lower entries, four per lane, with their source
+  int32_t low[4];
+  uint32_t lbase[4], lpos[4];
+  int lbits[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t e = lane + 64u * r;
+    const int k = lane_search(NL, e);
+    const uint32_t nlk = __shfl(NL, k);
+    const uint32_t o0k = __shfl(o0, k);
+    lbase[r] = __shfl(TB, k);
+    lbits[r] = __shfl(bits, k);
+    lpos[r] = e - nlk;
+    low[r] = e < nl_tot ? adj[o0k + lpos[r]] : -1;
+  }
+  for (uint32_t i = lane; i < tbl_tot; i += 64) keys[i] = -1;
   if (kSlots) {
-    for (uint32_t i = lane; i < nl; i += 64) ctr[i] = 0;
+    for (uint32_t i = lane; i < nl_tot; i += 64) ctr[i] = 0;
     reg[lane] = 0;
   }
   wave_lds_sync();
-  lds_insert(keys, kpos, bits, mask, adj, sb, nl, lane, 64);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    if (low[r] >= 0) {
+      const uint32_t mask = (1u << lbits[r]) - 1;
+      uint32_t hh = hash_slot(low[r], lbits[r]);
+      while (atomicCAS(&keys[lbase[r] + hh], -1, low[r]) != -1) hh = (hh + 1) & mask;
+      kpos[lbase[r] + hh] = static_cast<uint16_t>(lpos[r]);
+    }
+  }
+  const uint32_t P = wave_excl_scan(len, lane);
+  const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
   wave_lds_sync();
-  const HashSet set{keys, kpos, bits, mask};
-  const uint32_t found = wave_probe_flat<U, kSlots>(adj, cnt, set, ctr, accfwd::kWaveCap, sb, P, cb, T, reg, lane);
+  uint32_t found = 0;
+  for (uint32_t e0 = 0; e0 < T; e0 += 64 * U) {
+    int32_t a[U];
+    uint32_t idx[U];
+    int kq[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t e = e0 + 64 * q + lane;
+      const int k = lane_search(P, e);
+      kq[q] = k;
+      idx[q] = static_cast<uint32_t>(__shfl(cb, k)) + (e - static_cast<uint32_t>(__shfl(P, k)));
+      a[q] = e < T ? adj[idx[q]] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t tb = __shfl(p_tb, kq[q]);
+      const int tbits = __shfl(p_bits, kq[q]);
+      const uint32_t nlb = __shfl(p_nl, kq[q]);
+      bool hit = false;
+      if (a[q] >= 0 && tbits > 0) {
+        const int pos = lds_find(keys + tb, kpos + tb, tbits, (1u << tbits) - 1, a[q]);
+        if (pos >= 0) {
+          if (kSlots) {
+            atomicAdd(&cnt[idx[q]], 1);
+            atomicAdd(&ctr[nlb + pos], 1);
+          }
+          hit = true;
+        }
+      }
+      if (kSlots)
+        count_round(reg, kq[q], hit, lane);
+      else
+        found += hit;
+    }
+  }
   if (!kSlots) {
     const unsigned long long w = wave_sum_u64(found);
     if (lane == 0 && w) atomicAdd(tri_part(total), w);
     return;
   }
   wave_lds_sync();
-  if (static_cast<uint32_t>(lane) < nup) cnt[u.j0 + lane] = reg[lane];
-  for (uint32_t i = lane; i < nl; i += 64) {
-    const int v = ctr[i];
-    if (v) atomicAdd(&cnt[sb + i], v);
+  if (probe) cnt[j] = reg[lane];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t e = lane + 64u * r;
+    const int k = lane_search(NL, e);
+    const uint32_t dst = static_cast<uint32_t>(__shfl(o0, k)) + (e - static_cast<uint32_t>(__shfl(NL, k)));
+    if (e < nl_tot) {
+      const int v = ctr[e];
+      if (v) atomicAdd(&cnt[dst], v);
+    }
   }
 }
 
 // The forward counting launch.  Tasks [0, n_block) are one workgroup unit
-// each, the following tasks kWaves wave units each; workgroup w runs tasks
+// each, the following tasks kWaves wave packs each; workgroup w runs tasks
 // w, w + gridDim.x, ... (one task per workgroup when the grid covers them, a
 // persistent grid otherwise).  The task index is workgroup-uniform, so the
 // barrier that separates consecutive tasks' use of LDS is reached by all.
 // kSlots false: the triangle total only, accumulated into the partial sums at total.
 template <int U, bool kSlots>
 __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
+                                                   const uint32_t *__restrict__ hs,
                                                    const int32_t *__restrict__ adj,
                                                    const uint32_t *__restrict__ rev,
                                                    const FwdUnit *__restrict__ block_units, uint32_t n_block,
@@ -460,7 +525,7 @@ __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ 
       fwd_block_unit<U, kSlots>(off, adj, rev, block_units[t], cnt, total, smem);
     } else {
       const uint32_t ui = (t - n_block) * kWaves + wave;
-      if (ui < n_wave) fwd_wave_unit<U, kSlots>(off, adj, rev, wave_units[ui], cnt, total, smem, wave);
+      if (ui < n_wave) fwd_pack_unit<U, kSlots>(off, hs, adj, rev, wave_units[ui], cnt, total, smem, wave);
     }
     __syncthreads();
   }
@@ -673,15 +738,15 @@ int launch_count_t(acc_graph *g, unsigned long long *total) {
   if (grid) {
     switch (g->fwd_unroll) {
       case 1:
-        hipLaunchKernelGGL((k_fwd_count<1, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+        hipLaunchKernelGGL((k_fwd_count<1, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_hs, g->d_adj, g->d_rev,
                            g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
       case 4:
-        hipLaunchKernelGGL((k_fwd_count<4, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+        hipLaunchKernelGGL((k_fwd_count<4, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_hs, g->d_adj, g->d_rev,
                            g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
       default:
-        hipLaunchKernelGGL((k_fwd_count<2, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_rev,
+        hipLaunchKernelGGL((k_fwd_count<2, kSlots>), dim3(grid), dim3(kWG), 0, st, g->d_off, g->d_hs, g->d_adj, g->d_rev,
                            g->d_fwd_block, g->n_fwd_block, g->d_fwd_wave, g->n_fwd_wave, g->d_cnt, total);
         break;
     }
