@@ -133,6 +133,9 @@ int launch_merge(acc_graph *g, int plus);
 // Builds the slot -> source array (acc_scan.hip).
 int acc_graph_ensure_src(acc_graph *g);
 
+constexpr int kScanCounters = 16;
+constexpr uint32_t kItem = 256;  // list positions per work item of the index path
+
 // One SCAN query as the batched SCAN kernels see it (acc_scan.hip): the
 // resident graph's arrays, the query's buffers and its exact eps = p/q, mu.
 struct ScanJob {
@@ -146,6 +149,7 @@ struct ScanJob {
   const int32_t *snbr;
   const uint32_t *sslot;
   uint32_t *send;        // per vertex: end of the possibly-similar prefix
+  int2 *work;            // work items (vertex, first list position) of the index path
   int32_t *core_cnt;
   int32_t *label;
   int32_t *parent;
@@ -153,7 +157,7 @@ struct ScanJob {
   int32_t *rmax;
   int32_t *cl;
   int32_t *cid;
-  unsigned long long *counters;  // core, cluster, hub, outlier, pairs, index-path sums (5..7)
+  unsigned long long *counters;  // core, cluster, hub, outlier, pairs, index-path sums (5..7), item counts (8, 9)
   int8_t *out_label;             // host-mapped copies of the results (NULL: none)
   int32_t *out_cid;
   unsigned long long *out_counters;
@@ -168,10 +172,20 @@ struct ScanJob {
   uint32_t sb0, sbn;  // workgroups of slot-sized launches
 };
 
+// A launch's job table, passed by value as a kernel argument (no copy to
+// device memory before the kernels): at most kMaxJobs jobs per launch.
+constexpr int kMaxJobs = 14;
+struct JobTable {
+  ScanJob jobs[kMaxJobs];
+  int32_t nj;
+};
+static_assert(sizeof(JobTable) <= 3800, "kernel arguments are limited to 4 KiB");
+
 // Per-query device buffers, grown on demand.
 struct ScanBuffers {
   uint8_t *sim = nullptr;
   uint32_t *send = nullptr;
+  int2 *work = nullptr;
   int32_t *core_cnt = nullptr;
   int32_t *label = nullptr;
   int32_t *parent = nullptr;
@@ -192,16 +206,16 @@ uint64_t buffer_bytes(uint32_t n, uint64_t m);
 int check_eps(int64_t eps_num, int64_t eps_den);
 ScanJob make_job(const acc_graph *g, const ScanBuffers &b, int64_t eps_num, int64_t eps_den, int32_t mu);
 void plan_jobs(ScanJob *jobs, int nj, int cus, uint32_t *vblocks, uint32_t *sblocks);
-// Enqueues the eight SCAN kernels over the nj jobs at d_jobs (plan_jobs'd copy).
-int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
-int launch_pairs(const ScanJob *d_jobs, int nj, uint32_t sblocks, hipStream_t s);
+// Enqueues the eight per-slot SCAN kernels over the jobs of jt (planned).
+int launch_jobs(const JobTable &jt, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
+int launch_pairs(const JobTable &jt, uint32_t sblocks, hipStream_t s);
 // Builds the similarity index of a resident graph (counts computed, slot
 // sources built); enqueued on the graph's context stream.
 int build_index(acc_graph *g);
 // Whether the index decides eps = p/q exactly (p, q < 2^26) on graph g.
 bool index_usable(const acc_graph *g, int64_t p, int64_t q);
 // The six index-path SCAN kernels over nj jobs whose skey is set.
-int launch_index_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
+int launch_index_jobs(const JobTable &jt, uint32_t vblocks, uint32_t sblocks, hipStream_t s);
 }  // namespace accscan
 
 // Kernel launchers (k_forward.hip).

@@ -24,6 +24,16 @@ namespace {
 
 constexpr int kScanWG = 256;
 
+__device__ __forceinline__ uint32_t wave_excl_scan32(uint32_t v, int lane) {
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x - v;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -72,6 +82,25 @@ __device__ __forceinline__ void uf_union(int32_t *parent, int32_t a, int32_t b) 
   }
 }
 
+// Unites a with every lane's v where `valid`, a being the same on every lane
+// (a wave-uniform call): the roots involved are linked under their minimum
+// m in one step each -- distinct roots, so the CASes rarely collide, where 64
+// separate unions through a's root would queue on it.  A root that stopped
+// being one meanwhile takes the general union.  Larger roots still go under
+// smaller ones.
+__device__ __forceinline__ void uf_union_wave(int32_t *parent, int32_t a, int32_t v, bool valid) {
+  const int32_t ra = uf_find(parent, a);
+  const int32_t rv = valid ? uf_find(parent, v) : INT32_MAX;
+  int32_t mn = rv < ra ? rv : ra;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t t = __shfl_xor(mn, o, 64);
+    mn = t < mn ? t : mn;
+  }
+  if (valid && rv != mn && atomicCAS(&parent[rv], rv, mn) != rv) uf_union(parent, rv, mn);
+  if ((threadIdx.x & 63) == 0 && ra != mn && atomicCAS(&parent[ra], ra, mn) != ra) uf_union(parent, ra, mn);
+}
+
 __global__ __launch_bounds__(256) void k_fill_src(const uint32_t *__restrict__ off, uint32_t n,
                                                   int32_t *__restrict__ src) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * 256u + threadIdx.x) >> 6);
@@ -88,34 +117,33 @@ __global__ __launch_bounds__(256) void k_fill_src(const uint32_t *__restrict__ o
 // slot-sized launch and [vb0, vb0 + vbn) of a vertex-sized one, and those
 // workgroups stride over its m slots or n vertices.  The job of a workgroup
 // is found by a binary search of the table with workgroup-uniform values.
-__device__ __forceinline__ ScanJob job_of(const ScanJob *__restrict__ jobs, int nj, bool slots, uint32_t *local,
-                                          uint32_t *count) {
+__device__ __forceinline__ ScanJob job_of(const JobTable &jt, bool slots, uint32_t *local, uint32_t *count) {
   const uint32_t blk = blockIdx.x;
-  int lo = 0, hi = nj;
+  int lo = 0, hi = jt.nj;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if ((slots ? jobs[mid].sb0 : jobs[mid].vb0) <= blk) lo = mid; else hi = mid;
+    if ((slots ? jt.jobs[mid].sb0 : jt.jobs[mid].vb0) <= blk) lo = mid; else hi = mid;
   }
-  const ScanJob J = jobs[lo];
+  const ScanJob J = jt.jobs[lo];
   *local = blk - (slots ? J.sb0 : J.vb0);
   *count = slots ? J.sbn : J.vbn;
   return J;
 }
 
-__global__ __launch_bounds__(256) void k_sj_init(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_init(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   const uint32_t t = lb * 256u + threadIdx.x;
   for (uint32_t u = t; u < J.n; u += nb * 256u) J.core_cnt[u] = 0;
-  if (t < 8) J.counters[t] = 0;
+  if (t < kScanCounters) J.counters[t] = 0;
 }
 
 // Similarity of every slot and per-vertex counts of similar neighbors.  Each
 // wave covers 64 consecutive slots; runs of equal source are reduced with a
 // ballot and added with one atomic per run.
-__global__ __launch_bounds__(kScanWG) void k_sj_sim(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(kScanWG) void k_sj_sim(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const ScanJob J = job_of(jt, true, &lb, &nb);
   const uint32_t *__restrict__ off = J.off;
   const int32_t *__restrict__ adj = J.adj;
   const int32_t *__restrict__ src = J.src;
@@ -156,9 +184,9 @@ __global__ __launch_bounds__(kScanWG) void k_sj_sim(const ScanJob *__restrict__ 
   }
 }
 
-__global__ __launch_bounds__(256) void k_sj_label(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_label(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
     J.label[u] = J.core_cnt[u] >= J.mu ? 1 : 0;
     J.parent[u] = static_cast<int32_t>(u);
@@ -169,9 +197,9 @@ __global__ __launch_bounds__(256) void k_sj_label(const ScanJob *__restrict__ jo
 }
 
 // ClusterCore: union over similar core-core edges (each undirected edge once).
-__global__ __launch_bounds__(256) void k_sj_union(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_union(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const ScanJob J = job_of(jt, true, &lb, &nb);
   for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
     const int32_t u = J.src[i];
     const int32_t v = J.adj[i];
@@ -181,9 +209,9 @@ __global__ __launch_bounds__(256) void k_sj_union(const ScanJob *__restrict__ jo
 
 // Points every core at its root.  Only roots are stored, and every pointer
 // leads to the root, so concurrent walks stay correct.
-__global__ __launch_bounds__(256) void k_sj_compress(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_compress(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u)
     if (J.label[u] == 1)
       __hip_atomic_store(&J.parent[u], uf_root(J.parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
@@ -191,9 +219,9 @@ __global__ __launch_bounds__(256) void k_sj_compress(const ScanJob *__restrict__
 }
 
 // CheckHub: per non-core vertex, the smallest and largest root among its core neighbors.
-__global__ __launch_bounds__(256) void k_sj_hub(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_hub(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const ScanJob J = job_of(jt, true, &lb, &nb);
   for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
     const int32_t a = J.src[i];
     const int32_t v = J.adj[i];
@@ -207,9 +235,9 @@ __global__ __launch_bounds__(256) void k_sj_hub(const ScanJob *__restrict__ jobs
 
 // Labels hubs, takes the minimum old id of every cluster and counts
 // cores, clusters, hubs and outliers (counters[0..3]).
-__global__ __launch_bounds__(256) void k_sj_finalize(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_finalize(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   unsigned long long cores = 0, clusters = 0, hubs = 0, outliers = 0;
   for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
     if (J.label[u] == 1) {
@@ -248,15 +276,16 @@ __global__ __launch_bounds__(256) void k_sj_finalize(const ScanJob *__restrict__
   }
 }
 
-// Cluster ids; with host-mapped outputs set, the labels (as int8), cluster
-// ids and the four counters go straight to host memory instead.
-__global__ __launch_bounds__(256) void k_sj_cid(const ScanJob *__restrict__ jobs, int nj) {
+// Cluster ids; with host-mapped outputs set, the labels (as int8), the
+// cores' cluster ids and the four counters go straight to host memory
+// instead (the host keeps every other entry of out_cid at -1).
+__global__ __launch_bounds__(256) void k_sj_cid(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   if (J.out_cid) {
     for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
       const int32_t l = J.label[u];
-      J.out_cid[u] = l == 1 ? J.cl[J.parent[u]] : -1;
+      if (l == 1) J.out_cid[u] = J.cl[J.parent[u]];
       J.out_label[u] = static_cast<int8_t>(l);
     }
     if (lb == 0 && threadIdx.x < 4) J.out_counters[threadIdx.x] = J.counters[threadIdx.x];
@@ -268,8 +297,8 @@ __global__ __launch_bounds__(256) void k_sj_cid(const ScanJob *__restrict__ jobs
     J.cid[u] = J.label[u] == 1 ? J.cl[J.parent[u]] : -1;
 }
 
-__global__ __launch_bounds__(64) void k_sj_zero_pairs(const ScanJob *__restrict__ jobs, int nj) {
-  if (static_cast<int>(threadIdx.x) < nj) jobs[threadIdx.x].counters[4] = 0;
+__global__ __launch_bounds__(64) void k_sj_zero_pairs(const JobTable jt) {
+  if (static_cast<int>(threadIdx.x) < jt.nj) jt.jobs[threadIdx.x].counters[4] = 0;
 }
 
 // Non-core memberships: (cluster id of core u, old id of v) for every similar
@@ -277,9 +306,9 @@ __global__ __launch_bounds__(64) void k_sj_zero_pairs(const ScanJob *__restrict_
 // counts them, capacity pairs_cap.  Every lane takes its position from the
 // same counter with an increment of 0 or 1, so positions are unique without a
 // lane-0 broadcast.
-__global__ __launch_bounds__(256) void k_sj_pairs(const ScanJob *__restrict__ jobs, int nj) {
+__global__ __launch_bounds__(256) void k_sj_pairs(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const ScanJob J = job_of(jt, true, &lb, &nb);
   for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
     if (!J.sim[i]) continue;
     const int32_t u = J.src[i];
@@ -333,8 +362,8 @@ __device__ __forceinline__ bool exact_sim(const ScanJob &J, uint32_t s, uint64_t
   return lhs >= rhs;
 }
 
-__global__ __launch_bounds__(64) void k_sx_zero(const ScanJob *__restrict__ jobs) {
-  if (threadIdx.x < 8) jobs[blockIdx.x].counters[threadIdx.x] = 0;
+__global__ __launch_bounds__(64) void k_sx_zero(const JobTable jt) {
+  if (threadIdx.x < kScanCounters) jt.jobs[blockIdx.x].counters[threadIdx.x] = 0;
 }
 
 // First position in [lo, hi) of the descending keys K whose key is < ke
@@ -348,119 +377,292 @@ __device__ __forceinline__ uint32_t key_bound(const float *K, uint32_t lo, uint3
   return lo;
 }
 
+// Appends `items` work items (u, 0), (u, kItem), ... per lane to J.work,
+// counted in *count: one atomic per wave, each lane writing at its prefix.
+__device__ __forceinline__ void append_items(const ScanJob &J, unsigned long long *count, uint32_t u,
+                                             uint32_t items) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t pre = wave_excl_scan32(items, lane);
+  const uint32_t tot = __builtin_amdgcn_readfirstlane(__shfl(pre + items, 63));
+  if (!tot) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(count, static_cast<unsigned long long>(tot));
+  base = __shfl(base, 0);
+  for (uint32_t k = 0; k < items; k++)
+    J.work[base + pre + k] = make_int2(static_cast<int32_t>(u), static_cast<int32_t>(k * kItem));
+}
+
+// Items of every core's possibly-similar prefix (counters[8]).
+__device__ __forceinline__ void union_items(const ScanJob &J, uint32_t u, bool core, uint32_t end) {
+  append_items(J, &J.counters[8], u, core ? (end + kItem - 1) / kItem : 0);
+}
+
 // Core check from the index, with ke = key(eps^2) and K = u's sorted keys:
-// (counters[7] receives the total length of the cores' prefixes)
 // the mu-th largest key decides unless it ties with ke (K[mu-1] > ke: at
 // least mu similar neighbors; < ke: fewer than mu), in which case the tie
 // block takes the exact test.  send[u] = the end of a core's possibly-similar
-// prefix (keys >= ke), 0 for non-cores.  Degree sums over cores and
-// non-cores go to counters[5], counters[6].
-__global__ __launch_bounds__(256) void k_sx_core(const ScanJob *__restrict__ jobs, int nj) {
+// prefix (keys >= ke), 0 for non-cores; the prefixes are cut into the union's
+// work items.  Degree sums over cores and non-cores go to counters[5] and
+// counters[6], the total prefix length to counters[7].
+__global__ __launch_bounds__(256) void k_sx_core(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, false, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   const float ke = sim_key(J.p2, J.q2);
   unsigned long long core_deg = 0, other_deg = 0, prefix = 0;
-  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
-    const uint32_t base = J.off[u];
-    const uint32_t d = J.off[u + 1] - base;
-    const float *K = J.skey + base;
-    bool core;
+  const int lane = threadIdx.x & 63;
+  // wave-uniform loop: the item append at its end is a wave operation that
+  // every lane reaches
+  for (uint32_t u0 = lb * 256u + (threadIdx.x & ~63u); u0 < J.n; u0 += nb * 256u) {
+    const uint32_t u = u0 + lane;
+    bool core = false;
     uint32_t end = 0;
-    int32_t count = 0;
-    if (J.mu <= 0) {
-      core = true;
-      end = key_bound(K, 0, d, ke, true);
-    } else if (d < static_cast<uint32_t>(J.mu)) {
-      core = false;
-    } else {
-      const uint32_t t = static_cast<uint32_t>(J.mu) - 1;
-      const float k = K[t];
-      if (k > ke) {
+    if (u < J.n) {
+      const uint32_t base = J.off[u];
+      const uint32_t d = J.off[u + 1] - base;
+      const float *K = J.skey + base;
+      int32_t count = 0;
+      if (J.mu <= 0) {
         core = true;
-        end = key_bound(K, t + 1, d, ke, true);
-      } else if (k < ke) {
-        core = false;
+        end = key_bound(K, 0, d, ke, true);
+      } else if (d >= static_cast<uint32_t>(J.mu)) {
+        const uint32_t t = static_cast<uint32_t>(J.mu) - 1;
+        const float k = K[t];
+        if (k > ke) {
+          core = true;
+          end = key_bound(K, t + 1, d, ke, true);
+        } else if (k == ke) {
+          const uint32_t lo = key_bound(K, 0, t, ke, false);
+          end = key_bound(K, t + 1, d, ke, true);
+          count = static_cast<int32_t>(lo);
+          for (uint32_t i = lo; i < end; i++)
+            if (exact_sim(J, J.sslot[base + i], static_cast<uint64_t>(d) + 1, J.snbr[base + i])) count++;
+          core = count >= J.mu;
+        }
+      }
+      // each array only where a later kernel reads it: parent, cl and send
+      // of cores, rmin / rmax of non-cores; core counts only for workspaces
+      if (!J.out_cid) J.core_cnt[u] = count;
+      J.label[u] = core ? 1 : 0;
+      if (core) {
+        J.parent[u] = static_cast<int32_t>(u);
+        J.cl[u] = INT32_MAX;
+        J.send[u] = end;
       } else {
-        const uint32_t lo = key_bound(K, 0, t, ke, false);
-        end = key_bound(K, t + 1, d, ke, true);
-        count = static_cast<int32_t>(lo);
-        for (uint32_t i = lo; i < end; i++)
-          if (exact_sim(J, J.sslot[base + i], static_cast<uint64_t>(d) + 1, J.snbr[base + i])) count++;
-        core = count >= J.mu;
+        end = 0;
+        J.rmin[u] = INT32_MAX;
+        J.rmax[u] = -1;
+      }
+      if (core) {
+        core_deg += d;
+        prefix += end;
+      } else {
+        other_deg += d;
       }
     }
-    J.core_cnt[u] = count;
-    J.label[u] = core ? 1 : 0;
-    J.parent[u] = static_cast<int32_t>(u);
-    J.rmin[u] = INT32_MAX;
-    J.rmax[u] = -1;
-    J.cl[u] = INT32_MAX;
-    J.send[u] = core ? end : 0;
-    if (core) {
-      core_deg += d;
-      prefix += end;
-    } else {
-      other_deg += d;
-    }
+    union_items(J, u, core, end);
   }
   core_deg = wave_sum_u64(core_deg);
   other_deg = wave_sum_u64(other_deg);
   prefix = wave_sum_u64(prefix);
-  if ((threadIdx.x & 63) == 0) {
+  if (lane == 0) {
     if (core_deg) atomicAdd(&J.counters[5], core_deg);
     if (other_deg) atomicAdd(&J.counters[6], other_deg);
     if (prefix) atomicAdd(&J.counters[7], prefix);
   }
 }
 
-// Union of core a with the neighbor at sorted position i of its
-// possibly-similar prefix, when that neighbor is a core v > a and the pair is
-// similar (exact test on a key tie).
-__device__ __forceinline__ void union_at(const ScanJob &J, float ke, int32_t a, uint32_t i) {
-  const int32_t v = J.snbr[i];
-  if (v <= a || J.label[v] != 1) return;
-  if (J.skey[i] == ke &&
-      !exact_sim(J, J.sslot[i], static_cast<uint64_t>(J.off[a + 1] - J.off[a]) + 1, v))
-    return;
-  uf_union(J.parent, a, v);
-}
+// Whether the index path's clustering kernels take work items (the lists
+// they touch hold under a quarter of the slots) or sweep every slot.
+__device__ __forceinline__ bool use_items(unsigned long long touched, uint64_t m) { return touched * 4 < m; }
 
-// ClusterCore from the index, one thread per sorted position (even load
-// whatever the degree skew), skipping positions outside a core's
-// possibly-similar prefix; nothing to do when the prefixes are empty
-// (counters[7]).
-__global__ __launch_bounds__(256) void k_sx_union(const ScanJob *__restrict__ jobs, int nj) {
+
+// ClusterCore from the index, in three steps after the idea of Afforest
+// (Sutton, Ben-Nun and Barak, IPDPS 2018): every core first links to the
+// first two similar cores in its prefix (k_sx_link); the root holding most
+// of 1024 sampled cores is taken as the giant cluster's (k_sx_giant); then
+// only cores outside that cluster unite with every similar core of their
+// prefix, in both directions (k_sx_union).  Similarity is symmetric, so an
+// edge whose one end sits in the giant cluster is processed from its other
+// end, and the clusters are those of uniting every similar core pair.
+__global__ __launch_bounds__(256) void k_sx_link(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
+  const ScanJob J = job_of(jt, false, &lb, &nb);
   if (J.counters[7] == 0) return;
   const float ke = sim_key(J.p2, J.q2);
-  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
-    const int32_t a = J.src[i];
-    if (J.label[a] != 1) continue;
-    if (static_cast<uint32_t>(i) - J.off[a] >= J.send[a]) continue;
-    union_at(J, ke, a, static_cast<uint32_t>(i));
+  for (uint32_t u = lb * 256u + threadIdx.x; u < J.n; u += nb * 256u) {
+    if (J.label[u] != 1) continue;
+    const uint32_t base = J.off[u];
+    const uint32_t end = base + min(J.send[u], 64u);
+    int linked = 0;
+    for (uint32_t i = base; i < end && linked < 2; i++) {
+      const int32_t v = J.snbr[i];
+      if (J.label[v] != 1) continue;
+      if (J.skey[i] == ke && !exact_sim(J, J.sslot[i], static_cast<uint64_t>(J.off[u + 1] - base) + 1, v)) continue;
+      uf_union(J.parent, static_cast<int32_t>(u), v);
+      linked++;
+    }
   }
 }
 
-// CheckHub, one thread per slot, from the side with fewer adjacency slots
-// (counters[5] over cores, counters[6] over non-cores): cores push their root
-// to non-core neighbors, or non-cores pull their core neighbors' roots; both
-// leave the smallest and largest root in rmin / rmax.  Nothing to do when
-// either side is empty.
-__global__ __launch_bounds__(256) void k_sx_hub(const ScanJob *__restrict__ jobs, int nj) {
+// One workgroup per job: the most frequent root among up to 1024 sampled
+// cores (an LDS hash table of counts) into counters[10]; ~0 with no core.
+__global__ __launch_bounds__(1024) void k_sx_giant(const JobTable jt) {
+  __shared__ int32_t keys[2048];
+  __shared__ int32_t cnts[2048];
+  __shared__ unsigned long long best[16];
+  const ScanJob &J = jt.jobs[blockIdx.x];
+  const int t = threadIdx.x;
+  for (int i = t; i < 2048; i += 1024) {
+    keys[i] = -1;
+    cnts[i] = 0;
+  }
+  __syncthreads();
+  if (J.counters[7] != 0 && J.n) {
+    const uint32_t u = static_cast<uint32_t>((static_cast<uint64_t>(t) * J.n) / 1024u);
+    if (J.label[u] == 1) {
+      const int32_t r = uf_root(J.parent, static_cast<int32_t>(u));
+      uint32_t h = (static_cast<uint32_t>(r) * 0x9E3779B1u) >> 21;
+      for (;;) {
+        const int32_t k = atomicCAS(&keys[h], -1, r);
+        if (k == -1 || k == r) {
+          atomicAdd(&cnts[h], 1);
+          break;
+        }
+        h = (h + 1) & 2047;
+      }
+    }
+  }
+  __syncthreads();
+  // (count, root) packed so that the maximum is the most frequent root
+  unsigned long long m = 0;
+  for (int i = t; i < 2048; i += 1024)
+    if (cnts[i]) {
+      const unsigned long long c = (static_cast<unsigned long long>(cnts[i]) << 32) | static_cast<uint32_t>(keys[i]);
+      m = c > m ? c : m;
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(m, o, 64);
+    m = x > m ? x : m;
+  }
+  if ((t & 63) == 0) best[t >> 6] = m;
+  __syncthreads();
+  if (t == 0) {
+    unsigned long long b = 0;
+    for (int i = 0; i < 16; i++) b = best[i] > b ? best[i] : b;
+    J.counters[10] = b ? (b & 0xffffffffull) : ~0ull;
+  }
+}
+
+// One wave per work item written by k_sx_core (up to kItem consecutive
+// positions of one core's prefix); items of cores already in the giant
+// cluster are skipped.  Each round of 64 positions unites the core with its
+// similar cores through one uf_union_wave.  Loops are wave-uniform.
+__global__ __launch_bounds__(256) void k_sx_union(const JobTable jt) {
   uint32_t lb, nb;
-  const ScanJob J = job_of(jobs, nj, true, &lb, &nb);
-  const bool push = J.counters[5] <= J.counters[6];
-  if ((push ? J.counters[5] : J.counters[6]) == 0) return;
-  for (uint64_t i = lb * 256ull + threadIdx.x; i < J.m; i += nb * 256ull) {
-    const int32_t a = J.src[i];
-    if ((J.label[a] == 1) != push) continue;
-    const int32_t w = J.adj[i];
-    if ((J.label[w] == 1) == push) continue;
-    const int32_t r = push ? J.parent[a] : J.parent[w];
-    const int32_t x = push ? w : a;
-    if (r < J.rmin[x]) atomicMin(&J.rmin[x], r);
-    if (r > J.rmax[x]) atomicMax(&J.rmax[x], r);
+  const ScanJob J = job_of(jt, true, &lb, &nb);
+  if (J.counters[7] == 0) return;
+  const float ke = sim_key(J.p2, J.q2);
+  const int32_t giant = static_cast<int32_t>(J.counters[10]);  // -1: none
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = __builtin_amdgcn_readfirstlane(lb * 4u + (threadIdx.x >> 6));
+  const uint32_t nw = nb * 4u;
+  const uint32_t items = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(J.counters[8]));
+  for (uint32_t it = gw; it < items; it += nw) {
+    const int2 w = J.work[it];
+    const int32_t a = __builtin_amdgcn_readfirstlane(w.x);
+    if (__builtin_amdgcn_readfirstlane(uf_root(J.parent, a)) == giant) continue;
+    const uint32_t start = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(w.y));
+    const uint32_t base = __builtin_amdgcn_readfirstlane(J.off[a]);
+    const uint32_t end = base + min(start + kItem, static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(J.send[a])));
+    const uint64_t da1 = static_cast<uint64_t>(J.off[a + 1] - base) + 1;
+    for (uint32_t i0 = base + start; i0 < end; i0 += 64) {
+      const uint32_t i = i0 + lane;
+      int32_t v = 0;
+      bool valid = false;
+      if (i < end) {
+        v = J.snbr[i];
+        valid = v != a && J.label[v] == 1 && (J.skey[i] != ke || exact_sim(J, J.sslot[i], da1, v));
+      }
+      uf_union_wave(J.parent, a, v, valid);
+    }
+  }
+}
+
+// CheckHub side: the side with fewer adjacency slots (counters[5] over
+// cores, counters[6] over non-cores); cores push their root to non-core
+// neighbors, or non-cores pull their core neighbors' roots.
+__device__ __forceinline__ bool hub_push(const ScanJob &J) { return J.counters[5] <= J.counters[6]; }
+
+// Points every core at its root (as k_sj_compress) and, when CheckHub takes
+// work items, writes the items of every adjacency on the chosen side
+// (counters[9]).  Wave-uniform loop.
+__global__ __launch_bounds__(256) void k_sx_compress(const JobTable jt) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jt, false, &lb, &nb);
+  const bool push = hub_push(J);
+  const unsigned long long side = push ? J.counters[5] : J.counters[6];
+  const bool items = side != 0 && use_items(side, J.m);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t u0 = lb * 256u + (threadIdx.x & ~63u); u0 < J.n; u0 += nb * 256u) {
+    const uint32_t u = u0 + lane;
+    const bool in = u < J.n;
+    const bool core = in && J.label[u] == 1;
+    if (core)
+      __hip_atomic_store(&J.parent[u], uf_root(J.parent, static_cast<int32_t>(u)), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    if (items)
+      append_items(J, &J.counters[9], u, in && core == push ? (J.off[u + 1] - J.off[u] + kItem - 1) / kItem : 0);
+  }
+}
+
+// CheckHub.  Item form: one wave per work item of the chosen side; slot
+// form: every slot, four independent slots per thread and round.  Both
+// leave the smallest and largest root of each non-core's core neighbors in
+// rmin / rmax.
+__device__ __forceinline__ void hub_at(const ScanJob &J, bool push, int32_t a, int32_t ra, int32_t x) {
+  if ((J.label[x] == 1) == push) return;
+  const int32_t r = push ? ra : J.parent[x];
+  const int32_t y = push ? x : a;
+  if (r < J.rmin[y]) atomicMin(&J.rmin[y], r);
+  if (r > J.rmax[y]) atomicMax(&J.rmax[y], r);
+}
+
+__global__ __launch_bounds__(256) void k_sx_hub(const JobTable jt) {
+  uint32_t lb, nb;
+  const ScanJob J = job_of(jt, true, &lb, &nb);
+  const bool push = hub_push(J);
+  const unsigned long long side = push ? J.counters[5] : J.counters[6];
+  if (side == 0) return;
+  if (use_items(side, J.m)) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t items = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(J.counters[9]));
+    const uint32_t nw = nb * 4u;
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(lb * 4u + (threadIdx.x >> 6)); it < items; it += nw) {
+      const int2 w = J.work[it];
+      const int32_t a = __builtin_amdgcn_readfirstlane(w.x);
+      const uint32_t base = J.off[a];
+      const uint32_t end = min(base + static_cast<uint32_t>(w.y) + kItem, J.off[a + 1]);
+      const int32_t ra = push ? J.parent[a] : 0;
+      for (uint32_t i = base + static_cast<uint32_t>(w.y) + lane; i < end; i += 64) hub_at(J, push, a, ra, J.adj[i]);
+    }
+    return;
+  }
+  const uint64_t stride = nb * 256ull;
+  for (uint64_t i0 = lb * 256ull + threadIdx.x; i0 < J.m; i0 += 4 * stride) {
+    int32_t a[4], w[4];
+    bool on[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const uint64_t i = i0 + q * stride;
+      a[q] = i < J.m ? J.src[i] : -1;
+      w[q] = i < J.m ? J.adj[i] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) on[q] = a[q] >= 0 && (J.label[a[q]] == 1) == push;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (on[q]) hub_at(J, push, a[q], push ? J.parent[a[q]] : 0, w[q]);
   }
 }
 
@@ -490,27 +692,29 @@ void plan_jobs(ScanJob *jobs, int nj, int cus, uint32_t *vblocks, uint32_t *sblo
   *sblocks = s;
 }
 
-int launch_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_sj_init, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_sim, dim3(sblocks), dim3(kScanWG), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_label, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_union, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_compress, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_hub, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+int launch_jobs(const JobTable &jt, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sj_init, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_sim, dim3(sblocks), dim3(kScanWG), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_label, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_union, dim3(sblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_compress, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_hub, dim3(sblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, jt);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
 
-int launch_index_jobs(const ScanJob *d_jobs, int nj, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_sx_zero, dim3(nj), dim3(64), 0, s, d_jobs);
-  hipLaunchKernelGGL(k_sx_core, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sx_union, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_compress, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sx_hub, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, d_jobs, nj);
+int launch_index_jobs(const JobTable &jt, uint32_t vblocks, uint32_t sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sx_zero, dim3(jt.nj), dim3(64), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_core, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_link, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_giant, dim3(jt.nj), dim3(1024), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_union, dim3(sblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_compress, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sx_hub, dim3(sblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_finalize, dim3(vblocks), dim3(256), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_cid, dim3(vblocks), dim3(256), 0, s, jt);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
@@ -582,10 +786,9 @@ int build_index(acc_graph *g) {
   return ACC_OK;
 }
 
-int launch_pairs(const ScanJob *d_jobs, int nj, uint32_t sblocks, hipStream_t s) {
-  if (nj > 64) return acc_set_error(ACC_EINVAL, "at most 64 jobs per pairs launch");
-  hipLaunchKernelGGL(k_sj_zero_pairs, dim3(1), dim3(64), 0, s, d_jobs, nj);
-  hipLaunchKernelGGL(k_sj_pairs, dim3(sblocks), dim3(256), 0, s, d_jobs, nj);
+int launch_pairs(const JobTable &jt, uint32_t sblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_sj_zero_pairs, dim3(1), dim3(64), 0, s, jt);
+  hipLaunchKernelGGL(k_sj_pairs, dim3(sblocks), dim3(256), 0, s, jt);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
@@ -597,8 +800,7 @@ struct acc_scan_ws {
   hipStream_t stream = nullptr;
   hipEvent_t ready = nullptr;  // counts of the graph are complete
   ScanBuffers buf;
-  ScanJob *d_job = nullptr;
-  ScanJob *h_job = nullptr;  // pinned staging of the job description
+  JobTable jt{};  // the last query as a one-job table
   uint32_t vblocks = 0, sblocks = 0;
   int2 *d_pairs = nullptr;
   uint64_t pairs_cap = 0;
@@ -613,8 +815,6 @@ void free_ws(acc_scan_ws *w) {
   (void)hipSetDevice(w->g->ctx->device);
   if (w->stream) (void)hipStreamSynchronize(w->stream);
   accscan::free_buffers(&w->buf);
-  if (w->d_job) (void)hipFree(w->d_job);
-  if (w->h_job) (void)hipHostFree(w->h_job);
   if (w->d_pairs) (void)hipFree(w->d_pairs);
   if (w->ready) (void)hipEventDestroy(w->ready);
   if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -626,7 +826,8 @@ void free_ws(acc_scan_ws *w) {
 namespace accscan {
 
 void free_buffers(ScanBuffers *b) {
-  for (void *p : {static_cast<void *>(b->sim), static_cast<void *>(b->send), static_cast<void *>(b->core_cnt),
+  for (void *p : {static_cast<void *>(b->sim), static_cast<void *>(b->send), static_cast<void *>(b->work),
+                  static_cast<void *>(b->core_cnt),
                   static_cast<void *>(b->label),
                   static_cast<void *>(b->parent), static_cast<void *>(b->rmin), static_cast<void *>(b->rmax),
                   static_cast<void *>(b->cl), static_cast<void *>(b->cid), static_cast<void *>(b->counters)})
@@ -660,7 +861,8 @@ int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m) {
   if (!rc) rc = alloc(&b->rmax, nc);
   if (!rc) rc = alloc(&b->cl, nc);
   if (!rc) rc = alloc(&b->cid, nc);
-  if (!rc) rc = alloc(&b->counters, 8);
+  if (!rc) rc = alloc(&b->counters, kScanCounters);
+  if (!rc) rc = alloc(&b->work, static_cast<uint64_t>(nc) + mc / kItem + 64);
   if (rc) {
     free_buffers(b);
     return rc;
@@ -672,7 +874,8 @@ int reserve_buffers(ScanBuffers *b, uint32_t n, uint64_t m) {
 
 uint64_t buffer_bytes(uint32_t n, uint64_t m) {
   const uint64_t nn = n ? n : 1, mm = m ? m : 1;
-  return mm + 9 * sizeof(int32_t) * nn + 8 * sizeof(unsigned long long);
+  return mm + 9 * sizeof(int32_t) * nn + sizeof(int2) * (nn + mm / kItem + 64) +
+         kScanCounters * sizeof(unsigned long long);
 }
 
 int check_eps(int64_t eps_num, int64_t eps_den) {
@@ -694,6 +897,7 @@ ScanJob make_job(const acc_graph *g, const ScanBuffers &b, int64_t eps_num, int6
   j.snbr = g->d_snbr;
   j.sslot = g->d_sslot;
   j.send = b.send;
+  j.work = b.work;
   j.core_cnt = b.core_cnt;
   j.label = b.label;
   j.parent = b.parent;
@@ -770,15 +974,7 @@ int acc_scan_ws_create(acc_graph *g, acc_scan_ws **out) {
   e = hipEventCreateWithFlags(&w->ready, hipEventDisableTiming);
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(e)));
   if ((rc = accscan::reserve_buffers(&w->buf, g->n, g->m))) return bail(rc);
-  if (hipMalloc(reinterpret_cast<void **>(&w->d_job), sizeof(ScanJob)) != hipSuccess) {
-    w->d_job = nullptr;
-    return bail(acc_set_error(ACC_ENOMEM, "hipMalloc of the SCAN job failed"));
-  }
-  if (hipHostMalloc(reinterpret_cast<void **>(&w->h_job), sizeof(ScanJob), hipHostMallocDefault) != hipSuccess) {
-    w->h_job = nullptr;
-    return bail(acc_set_error(ACC_ENOMEM, "hipHostMalloc of the SCAN job failed"));
-  }
-  w->device_bytes = w->buf.bytes + sizeof(ScanJob);
+  w->device_bytes = w->buf.bytes;
   if ((rc = acc_graph_ensure_src(g))) return bail(rc);
   *out = w;
   return ACC_OK;
@@ -802,12 +998,10 @@ int acc_scan_run(acc_scan_ws *w, int64_t eps_num, int64_t eps_den, int32_t mu, u
   // order this query after the graph's counts
   ACC_HIP_TRY(hipEventRecord(w->ready, g->ctx->stream));
   ACC_HIP_TRY(hipStreamWaitEvent(w->stream, w->ready, 0));
-  // the staging of the previous query's job may still be in flight
-  ACC_HIP_TRY(hipStreamSynchronize(w->stream));
-  *w->h_job = accscan::make_job(g, w->buf, eps_num, eps_den, mu);
-  accscan::plan_jobs(w->h_job, 1, g->ctx->num_cus, &w->vblocks, &w->sblocks);
-  ACC_HIP_TRY(hipMemcpyAsync(w->d_job, w->h_job, sizeof(ScanJob), hipMemcpyHostToDevice, w->stream));
-  ACC_TRY(accscan::launch_jobs(w->d_job, 1, w->vblocks, w->sblocks, w->stream));
+  w->jt.jobs[0] = accscan::make_job(g, w->buf, eps_num, eps_den, mu);
+  w->jt.nj = 1;
+  accscan::plan_jobs(w->jt.jobs, 1, g->ctx->num_cus, &w->vblocks, &w->sblocks);
+  ACC_TRY(accscan::launch_jobs(w->jt, w->vblocks, w->sblocks, w->stream));
   (void)flags;
   w->have_result = true;
   return ACC_OK;
@@ -857,11 +1051,9 @@ int acc_scan_pairs(acc_scan_ws *w, int32_t *pairs, uint64_t cap, uint64_t *count
     w->device_bytes += bytes;
     w->pairs_cap = g->m;
   }
-  ACC_HIP_TRY(hipStreamSynchronize(s));
-  w->h_job->pairs = w->d_pairs;
-  w->h_job->pairs_cap = w->pairs_cap;
-  ACC_HIP_TRY(hipMemcpyAsync(w->d_job, w->h_job, sizeof(ScanJob), hipMemcpyHostToDevice, s));
-  ACC_TRY(accscan::launch_pairs(w->d_job, 1, w->sblocks, s));
+  w->jt.jobs[0].pairs = w->d_pairs;
+  w->jt.jobs[0].pairs_cap = w->pairs_cap;
+  ACC_TRY(accscan::launch_pairs(w->jt, w->sblocks, s));
   unsigned long long k = 0;
   ACC_HIP_TRY(hipMemcpyAsync(&k, w->buf.counters + 4, sizeof(k), hipMemcpyDeviceToHost, s));
   ACC_HIP_TRY(hipStreamSynchronize(s));

@@ -84,8 +84,8 @@ struct Slot {
 struct Batch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  ScanJob *d_jobs = nullptr;
-  ScanJob *h_jobs = nullptr;  // pinned staging
+  JobTable index_jobs{};  // queries the similarity index decides
+  JobTable slot_jobs{};   // the others
   std::vector<uint64_t> tickets;
   bool busy = false;
   std::thread waiter;
@@ -235,13 +235,14 @@ int reserve_slot(acc_server *s, Slot &sl, uint32_t n, uint64_t m) {
         hipHostMalloc(reinterpret_cast<void **>(&sl.h_cid), sizeof(int32_t) * k, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
       return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result buffers failed");
     if (!sl.h_counters &&
-        hipHostMalloc(reinterpret_cast<void **>(&sl.h_counters), 8 * sizeof(unsigned long long),
+        hipHostMalloc(reinterpret_cast<void **>(&sl.h_counters), kScanCounters * sizeof(unsigned long long),
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
       return acc_set_error(ACC_ENOMEM, "hipHostMalloc of result counters failed");
     if (hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_label), sl.h_label, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_cid), sl.h_cid, 0) != hipSuccess ||
         hipHostGetDevicePointer(reinterpret_cast<void **>(&sl.d_counters), sl.h_counters, 0) != hipSuccess)
       return acc_set_error(ACC_EDEVICE, "device address of result buffers unavailable");
+    std::fill(sl.h_cid, sl.h_cid + k, -1);  // kernels write only the cores' entries
     sl.h_cap = static_cast<uint32_t>(k);
   }
   return ACC_OK;
@@ -393,19 +394,21 @@ bool launch_batch(acc_server *s) {
     Query &q = s->queries[take[j]];
     q.slot = slot_of[j];
     const Slot &sl = s->slots[q.slot];
-    ScanJob &job = b->h_jobs[j];
+    ScanJob &job = j < ni ? b->index_jobs.jobs[j] : b->slot_jobs.jobs[j - ni];
     job = accscan::make_job(s->graphs[q.gid].g, sl.buf, q.p, q.q, q.mu);
     job.out_label = sl.d_label;
     job.out_cid = sl.d_cid;
     job.out_counters = sl.d_counters;
   }
   uint32_t vb_i = 0, sb_i = 0, vblocks = 0, sblocks = 0;
-  accscan::plan_jobs(b->h_jobs, ni, s->ctx->num_cus, &vb_i, &sb_i);
-  accscan::plan_jobs(b->h_jobs + ni, nj - ni, s->ctx->num_cus, &vblocks, &sblocks);
-  hipError_t e = hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(ScanJob) * nj, hipMemcpyHostToDevice, b->stream);
-  int rc = e == hipSuccess ? ACC_OK : acc_set_error(ACC_EDEVICE, std::string("job upload: ") + hipGetErrorString(e));
-  if (!rc && ni) rc = accscan::launch_index_jobs(b->d_jobs, ni, vb_i, sb_i, b->stream);
-  if (!rc && nj > ni) rc = accscan::launch_jobs(b->d_jobs + ni, nj - ni, vblocks, sblocks, b->stream);
+  b->index_jobs.nj = ni;
+  b->slot_jobs.nj = nj - ni;
+  accscan::plan_jobs(b->index_jobs.jobs, ni, s->ctx->num_cus, &vb_i, &sb_i);
+  accscan::plan_jobs(b->slot_jobs.jobs, nj - ni, s->ctx->num_cus, &vblocks, &sblocks);
+  hipError_t e = hipSuccess;
+  int rc = ACC_OK;
+  if (ni) rc = accscan::launch_index_jobs(b->index_jobs, vb_i, sb_i, b->stream);
+  if (!rc && nj > ni) rc = accscan::launch_jobs(b->slot_jobs, vblocks, sblocks, b->stream);
   if (!rc && (e = hipEventRecord(b->done, b->stream)) != hipSuccess)
     rc = acc_set_error(ACC_EDEVICE, std::string("hipEventRecord: ") + hipGetErrorString(e));
   if (rc) {
@@ -456,8 +459,6 @@ void destroy_server(acc_server *s) {
   (void)hipSetDevice(s->ctx->device);
   for (Batch &b : s->batches) {
     if (b.stream) (void)hipStreamSynchronize(b.stream);
-    if (b.d_jobs) (void)hipFree(b.d_jobs);
-    if (b.h_jobs) (void)hipHostFree(b.h_jobs);
     if (b.done) (void)hipEventDestroy(b.done);
     if (b.stream) (void)hipStreamDestroy(b.stream);
   }
@@ -474,7 +475,7 @@ extern "C" {
 void acc_server_config_default(acc_server_config *cfg) {
   if (!cfg) return;
   cfg->hbm_budget = 64ull << 30;
-  cfg->max_batch = 16;
+  cfg->max_batch = kMaxJobs;
   cfg->streams = 4;
   cfg->slots = 32;
   cfg->no_index = 0;
@@ -486,8 +487,9 @@ int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **o
   acc_server_config c;
   acc_server_config_default(&c);
   if (cfg) c = *cfg;
-  if (c.max_batch < 1 || c.max_batch > 256 || c.streams < 1 || c.streams > 16 || c.slots < 1 || c.slots > 4096)
-    return acc_set_error(ACC_EINVAL, "server config: max_batch in [1,256], streams in [1,16], slots in [1,4096]");
+  if (c.max_batch < 1 || c.max_batch > kMaxJobs || c.streams < 1 || c.streams > 16 || c.slots < 1 || c.slots > 4096)
+    return acc_set_error(ACC_EINVAL, "server config: max_batch in [1," + std::to_string(kMaxJobs) +
+                                         "], streams in [1,16], slots in [1,4096]");
   (void)hipSetDevice(ctx->device);
   auto *s = new (std::nothrow) acc_server;
   if (!s) return acc_set_error(ACC_ENOMEM, "host allocation failed");
@@ -498,9 +500,6 @@ int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **o
   for (Batch &b : s->batches) {
     hipError_t e = hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&b.d_jobs), sizeof(ScanJob) * c.max_batch);
-    if (e == hipSuccess)
-      e = hipHostMalloc(reinterpret_cast<void **>(&b.h_jobs), sizeof(ScanJob) * c.max_batch, hipHostMallocDefault);
     if (e != hipSuccess) {
       destroy_server(s);
       return acc_set_error(ACC_EDEVICE, std::string("server setup: ") + hipGetErrorString(e));
@@ -532,6 +531,11 @@ int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *
     return rc;
   }
   s->graphs[gid].last_use = ++s->clock;
+  // Grow the idle result slots for this graph now, so that queries do not
+  // allocate device or pinned memory while they are served (as far as the
+  // budget allows; a slot that cannot grow here grows on first use).
+  for (Slot &sl : s->slots)
+    if (!sl.busy && reserve_slot(s, sl, n, m) != ACC_OK) break;
   *graph_id = gid;
   return ACC_OK;
 }
@@ -632,20 +636,42 @@ int acc_server_result(acc_server *s, uint64_t ticket, const int8_t **label, cons
 
 int acc_server_release(acc_server *s, uint64_t ticket) {
   if (!s) return acc_set_error(ACC_EINVAL, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->queries.find(ticket);
-  if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
-  Query &q = it->second;
-  if (q.state == kPending || q.state == kRunning) {
+  int slot = -1;
+  uint32_t n = 0;
+  bool whole = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->queries.find(ticket);
+    if (it == s->queries.end()) return acc_set_error(ACC_ENOTFOUND, "unknown ticket");
+    Query &q = it->second;
     if (q.state == kRunning) return acc_set_error(ACC_EINVAL, "query is running");
-    // a pending query is cancelled
-    auto pi = std::find(s->pending.begin(), s->pending.end(), ticket);
-    if (pi != s->pending.end()) s->pending.erase(pi);
-    s->graphs[q.gid].waiting--;
-    s->st.cancelled++;
+    if (q.state == kPending) {  // a pending query is cancelled
+      auto pi = std::find(s->pending.begin(), s->pending.end(), ticket);
+      if (pi != s->pending.end()) s->pending.erase(pi);
+      s->graphs[q.gid].waiting--;
+      s->st.cancelled++;
+    }
+    slot = q.slot;
+    if (slot >= 0) {
+      n = std::min(s->graphs[q.gid].n, s->slots[slot].h_cap);
+      whole = q.state == kFailed;
+    }
+    s->queries.erase(it);
   }
-  if (q.slot >= 0) s->slots[q.slot].busy = false;
-  s->queries.erase(it);
+  if (slot >= 0) {
+    // Back to -1 everywhere before the slot is reused: the next query's
+    // kernel writes only its cores' cluster ids.  The slot stays busy (owned
+    // by this call) meanwhile, so this runs without the lock.
+    Slot &sl = s->slots[slot];
+    if (whole) {
+      std::fill(sl.h_cid, sl.h_cid + sl.h_cap, -1);
+    } else {
+      for (uint32_t u = 0; u < n; u++)
+        if (sl.h_label[u] == 1) sl.h_cid[u] = -1;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    sl.busy = false;
+  }
   s->cv_work.notify_one();
   return ACC_OK;
 }

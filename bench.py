@@ -190,7 +190,7 @@ def bench_batch(ctx, steps: int, warmup: int, num_graphs: int = 512):
     }
 
 
-def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16, seed: int = 5):
+def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16, seed: int = 5, streams: int = 4):
     """Serving form: `graphs` resident R-MAT graphs (scale uniform in 16..22,
     edge factor 16, degree-relabelled), SCAN queries (graph uniform, eps
     uniform in {0.1, ..., 0.9}, mu uniform in {2, 5, 10, 20}) arriving as a
@@ -209,7 +209,7 @@ def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16
     gs = [G.gen_rmat(sc, 16, seed=1000 + i) for i, sc in enumerate(scales)]
     print(f"[serve] {graphs} R-MAT graphs, scales {scales}, {sum(g.m for g in gs)} slots, "
           f"gen {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    sv = serve.Server(ctx, max_batch=16, streams=4, slots=64)
+    sv = serve.Server(ctx, max_batch=14, streams=streams, slots=64)
     t0 = time.perf_counter()
     gids = [sv.add_graph(g) for g in gs]
     t_admit = time.perf_counter() - t0
@@ -269,8 +269,11 @@ def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16
         same = same and bool(np.array_equal(a["label"], r["label"]) and np.array_equal(a["cluster_id"],
                                                                                          r["cluster_id"]))
     batches = after["batches"] - before["batches"]
+    sc = np.array([scales[k] for k, _, _ in plan])
+    by_scale = {int(x): round(float(np.percentile(lat[sc == x], 99)), 3) for x in sorted(set(scales))}
+    slow = lat > np.percentile(lat, 99)
     out = {
-        "graphs": graphs, "scales": scales, "queries": queries, "arrival_rate_qps": rate,
+        "graphs": graphs, "scales": scales, "queries": queries, "arrival_rate_qps": rate, "streams": streams,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "max_ms": float(lat.max()), "mean_ms": float(lat.mean()),
         "queries_per_s": queries / wall, "batches": batches,
@@ -278,6 +281,9 @@ def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16
         "failed": after["failed"] - before["failed"], "admit_s": t_admit,
         "resident_gb": after["graph_bytes"] / 1e9, "slot_gb": after["slot_bytes"] / 1e9,
         "identical_alone_sampled": same, "sampled": len(kept),
+        "p99_ms_by_scale": by_scale,
+        "slowest_1pct_by_scale": {int(x): int((sc[slow] == x).sum()) for x in sorted(set(sc[slow].tolist()))},
+        "slowest_1pct_eps": sorted(set(plan[i][1] for i in np.nonzero(slow)[0])),
     }
     sv.close()
     print(f"[serve] p50 {out['p50_ms']:.3f} ms p99 {out['p99_ms']:.3f} ms, {out['queries_per_s']:.0f} q/s, "
@@ -301,6 +307,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
     ap.add_argument("--serve-queries", type=int, default=6000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -367,7 +374,8 @@ def main():
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
     rg.release()
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
-    served = None if args.no_serve else bench_serve(ctx, args.serve_queries, args.serve_rate)
+    served = None if args.no_serve else bench_serve(ctx, args.serve_queries, args.serve_rate,
+                                                     streams=args.serve_streams)
     ctx.close()
 
     cpu = None if args.no_cpu else cpu_baseline(g, args.cpu_seconds)

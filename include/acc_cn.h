@@ -190,7 +190,7 @@ int acc_scan_pairs(acc_scan_ws *ws, int32_t *pairs, uint64_t cap, uint64_t *coun
 typedef struct acc_server acc_server;
 typedef struct acc_server_config {
   uint64_t hbm_budget; /* device bytes for resident graphs plus result slots */
-  int32_t max_batch;   /* queries per batched launch, 1..256 */
+  int32_t max_batch;   /* queries per batched launch, 1..14 (the job table is a kernel argument) */
   int32_t streams;     /* batches in flight at once, 1..16 */
   int32_t slots;       /* queries holding results at once (launched, not released), 1..4096 */
   int32_t no_index;    /* 1: build no similarity index (every query takes the per-slot kernels) */
@@ -208,11 +208,13 @@ typedef struct acc_server_stats {
 #define ACC_QUERY_DONE 2
 #define ACC_QUERY_FAILED 3
 
-void acc_server_config_default(acc_server_config *cfg); /* 64 GiB, batch 16, 4 streams, 32 slots */
+void acc_server_config_default(acc_server_config *cfg); /* 64 GiB, batch 14, 4 streams, 32 slots */
 int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **out);
 void acc_server_destroy(acc_server *s);
 /* Registers a CSR (copied) and admits it; the first admission validates it
- * unless flags has ACC_UPLOAD_NO_VALIDATE.  ACC_EBUDGET if it cannot fit. */
+ * unless flags has ACC_UPLOAD_NO_VALIDATE.  ACC_EBUDGET if it cannot fit.
+ * Idle result slots are grown for the graph here, within the budget, so
+ * that serving queries allocates nothing. */
 int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
                          uint32_t flags, int32_t *graph_id);
 int acc_server_remove_graph(acc_server *s, int32_t graph_id);
