@@ -190,7 +190,8 @@ def bench_batch(ctx, steps: int, warmup: int, num_graphs: int = 512):
     }
 
 
-def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16, seed: int = 5, streams: int = 4):
+def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16, seed: int = 5, streams: int = 4,
+                extra_rates=()):
     """Serving form: `graphs` resident R-MAT graphs (scale uniform in 16..22,
     edge factor 16, degree-relabelled), SCAN queries (graph uniform, eps
     uniform in {0.1, ..., 0.9}, mu uniform in {2, 5, 10, 20}) arriving as a
@@ -221,70 +222,83 @@ def bench_serve(ctx, queries: int = 6000, rate: float = 2000.0, graphs: int = 16
     for t in warm:
         sv.wait(t)
         sv.release(t)
-    plan = [(int(rng.integers(graphs)), epss[int(rng.integers(9))], mus[int(rng.integers(4))])
-            for _ in range(queries)]
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=queries))
-    keep = set(int(i) for i in rng.choice(queries, size=min(48, queries), replace=False))
-    tickets = [None] * queries
-    lat = np.zeros(queries)
-    kept = {}
-    ready = threading.Semaphore(0)
+    def run(rate, queries, rng):
+        plan = [(int(rng.integers(graphs)), epss[int(rng.integers(9))], mus[int(rng.integers(4))])
+                for _ in range(queries)]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=queries))
+        keep = set(int(i) for i in rng.choice(queries, size=min(48, queries), replace=False))
+        tickets = [None] * queries
+        lat = np.zeros(queries)
+        kept = {}
+        ready = threading.Semaphore(0)
 
-    def collect():
-        for i in range(queries):
-            ready.acquire()
-            t = tickets[i]
-            sv.wait(t)
-            r = sv.result(t, copy=i in keep)
-            lat[i] = r["latency_ms"]
-            if i in keep:
-                kept[i] = r
-            sv.release(t)
+        def collect():
+            for i in range(queries):
+                ready.acquire()
+                t = tickets[i]
+                sv.wait(t)
+                r = sv.result(t, copy=i in keep)
+                lat[i] = r["latency_ms"]
+                if i in keep:
+                    kept[i] = r
+                sv.release(t)
 
-    before = sv.stats()
-    # the submitting loop sleeps between arrivals (a spin would hold the
-    # interpreter lock and starve the collecting thread that frees slots)
-    switch = sys.getswitchinterval()
-    sys.setswitchinterval(1e-4)
-    th = threading.Thread(target=collect)
-    th.start()
-    start = time.perf_counter()
-    for i, (k, eps, mu) in enumerate(plan):
-        wait = arrivals[i] - (time.perf_counter() - start)
-        if wait > 0:
-            time.sleep(wait)
-        tickets[i] = sv.submit(gids[k], eps, mu)
-        ready.release()
-    th.join()
-    sys.setswitchinterval(switch)
-    wall = time.perf_counter() - start
-    after = sv.stats()
-    same = True
-    for i, r in kept.items():
-        k, eps, mu = plan[i]
-        with ctx.upload(gs[k]) as rg, rg.scan_workspace() as ws:
-            ws.run(eps, mu)
-            ws.synchronize()
-            a = ws.results()
-        same = same and bool(np.array_equal(a["label"], r["label"]) and np.array_equal(a["cluster_id"],
-                                                                                         r["cluster_id"]))
-    batches = after["batches"] - before["batches"]
-    sc = np.array([scales[k] for k, _, _ in plan])
-    by_scale = {int(x): round(float(np.percentile(lat[sc == x], 99)), 3) for x in sorted(set(scales))}
-    slow = lat > np.percentile(lat, 99)
-    out = {
-        "graphs": graphs, "scales": scales, "queries": queries, "arrival_rate_qps": rate, "streams": streams,
-        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-        "max_ms": float(lat.max()), "mean_ms": float(lat.mean()),
-        "queries_per_s": queries / wall, "batches": batches,
-        "mean_batch": (after["batched_queries"] - before["batched_queries"]) / max(batches, 1),
-        "failed": after["failed"] - before["failed"], "admit_s": t_admit,
-        "resident_gb": after["graph_bytes"] / 1e9, "slot_gb": after["slot_bytes"] / 1e9,
-        "identical_alone_sampled": same, "sampled": len(kept),
-        "p99_ms_by_scale": by_scale,
-        "slowest_1pct_by_scale": {int(x): int((sc[slow] == x).sum()) for x in sorted(set(sc[slow].tolist()))},
-        "slowest_1pct_eps": sorted(set(plan[i][1] for i in np.nonzero(slow)[0])),
-    }
+        before = sv.stats()
+        # the submitting loop sleeps between arrivals (a spin would hold the
+        # interpreter lock and starve the collecting thread that frees slots)
+        switch = sys.getswitchinterval()
+        sys.setswitchinterval(1e-4)
+        th = threading.Thread(target=collect)
+        th.start()
+        start = time.perf_counter()
+        for i, (k, eps, mu) in enumerate(plan):
+            wait = arrivals[i] - (time.perf_counter() - start)
+            if wait > 0:
+                time.sleep(wait)
+            tickets[i] = sv.submit(gids[k], eps, mu)
+            ready.release()
+        th.join()
+        sys.setswitchinterval(switch)
+        wall = time.perf_counter() - start
+        after = sv.stats()
+        same = True
+        for i, r in kept.items():
+            k, eps, mu = plan[i]
+            with ctx.upload(gs[k]) as rg, rg.scan_workspace() as ws:
+                ws.run(eps, mu)
+                ws.synchronize()
+                a = ws.results()
+            same = same and bool(np.array_equal(a["label"], r["label"]) and np.array_equal(a["cluster_id"],
+                                                                                             r["cluster_id"]))
+        batches = after["batches"] - before["batches"]
+        sc = np.array([scales[k] for k, _, _ in plan])
+        by_scale = {int(x): round(float(np.percentile(lat[sc == x], 99)), 3) for x in sorted(set(scales))}
+        slow = lat > np.percentile(lat, 99)
+        out = {
+            "graphs": graphs, "scales": scales, "queries": queries, "arrival_rate_qps": rate, "streams": streams,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max()), "mean_ms": float(lat.mean()),
+            "queries_per_s": queries / wall, "batches": batches,
+            "mean_batch": (after["batched_queries"] - before["batched_queries"]) / max(batches, 1),
+            "failed": after["failed"] - before["failed"], "admit_s": t_admit,
+            "resident_gb": after["graph_bytes"] / 1e9, "slot_gb": after["slot_bytes"] / 1e9,
+            "identical_alone_sampled": same, "sampled": len(kept),
+            "p99_ms_by_scale": by_scale,
+            "slowest_1pct_by_scale": {int(x): int((sc[slow] == x).sum()) for x in sorted(set(sc[slow].tolist()))},
+            "slowest_1pct_eps": sorted(set(plan[i][1] for i in np.nonzero(slow)[0])),
+        }
+
+        return out
+
+    out = run(rate, queries, rng)
+    extra = []
+    for r2 in [r for r in extra_rates if r > 0]:
+        o2 = run(r2, queries, np.random.default_rng(seed + int(r2)))
+        extra.append({k: o2[k] for k in ("arrival_rate_qps", "p50_ms", "p99_ms", "max_ms", "queries_per_s",
+                                         "mean_batch", "failed", "identical_alone_sampled")})
+        print(f"[serve] at {r2:.0f} q/s: p50 {o2['p50_ms']:.3f} ms p99 {o2['p99_ms']:.3f} ms, "
+              f"{o2['queries_per_s']:.0f} q/s, mean batch {o2['mean_batch']:.2f}", file=sys.stderr, flush=True)
+    out["higher_rates"] = extra
     sv.close()
     print(f"[serve] p50 {out['p50_ms']:.3f} ms p99 {out['p99_ms']:.3f} ms, {out['queries_per_s']:.0f} q/s, "
           f"mean batch {out['mean_batch']:.2f}", file=sys.stderr, flush=True)
@@ -308,6 +322,8 @@ def main():
     ap.add_argument("--serve-queries", type=int, default=6000)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-rate-high", type=float, action="append", default=None,
+                    help="also measure at these arrival rates (default 4000 q/s; 0 for none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -374,8 +390,9 @@ def main():
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
     rg.release()
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
-    served = None if args.no_serve else bench_serve(ctx, args.serve_queries, args.serve_rate,
-                                                     streams=args.serve_streams)
+    served = None if args.no_serve else bench_serve(
+        ctx, args.serve_queries, args.serve_rate, streams=args.serve_streams,
+        extra_rates=args.serve_rate_high if args.serve_rate_high is not None else [4000.0])
     ctx.close()
 
     cpu = None if args.no_cpu else cpu_baseline(g, args.cpu_seconds)
