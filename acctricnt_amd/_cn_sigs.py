@@ -35,6 +35,7 @@ SIGNATURES = {
     "acc_graph_upload": (C.c_int, [vp, C.c_uint32, C.c_uint64, u32p, i32p, C.c_uint32, C.POINTER(vp)]),
     "acc_graph_release": (None, [vp]),
     "acc_graph_info": (C.c_int, [vp, u32p, u64p, u64p]),
+    "acc_graph_download": (C.c_int, [vp, u32p, i32p, i32p]),
     "acc_cn_compute": (C.c_int, [vp, C.c_int, C.c_uint32]),
     "acc_cn_download": (C.c_int, [vp, i32p, u64p]),
     "acc_cn_all_edges": (C.c_int, [vp, C.c_int, C.c_uint32, i32p, u64p]),

@@ -33,6 +33,7 @@ TUNE_FORWARD_ONLY = 0x2
 CN_RAW = 0x1
 CN_TIMING = 0x2
 UPLOAD_NO_VALIDATE = 0x1
+UPLOAD_RELABEL = 0x2
 
 
 def _path(p) -> int:
@@ -67,8 +68,11 @@ class Context:
     def stream(self) -> int:
         return self._lib.acc_ctx_stream(self._h) or 0
 
-    def upload(self, graph, validate: bool = True) -> "ResidentGraph":
-        rg = ResidentGraph(self, graph, validate)
+    def upload(self, graph, validate: bool = True, relabel: bool = False) -> "ResidentGraph":
+        """Copy a CSR into HBM; relabel=True relabels it on the device by
+        degree, descending, ties by ascending id (read it back with
+        :meth:`ResidentGraph.csr`)."""
+        rg = ResidentGraph(self, graph, validate, relabel)
         self._graphs.add(rg)
         return rg
 
@@ -97,7 +101,7 @@ class Context:
 class ResidentGraph:
     """A CSR resident in HBM (``acc_graph``) plus its count array."""
 
-    def __init__(self, ctx: Context, graph, validate: bool = True):
+    def __init__(self, ctx: Context, graph, validate: bool = True, relabel: bool = False):
         self.ctx = ctx
         self._lib = ctx._lib
         self.n = graph.n
@@ -107,7 +111,7 @@ class ResidentGraph:
         if off.shape[0] != self.n + 1:
             raise ValueError("node_off must have n+1 entries")
         h = C.c_void_p()
-        flags = 0 if validate else UPLOAD_NO_VALIDATE
+        flags = (0 if validate else UPLOAD_NO_VALIDATE) | (UPLOAD_RELABEL if relabel else 0)
         check_cn(self._lib.acc_graph_upload(ctx.handle, self.n, self.m, _ffi.ptr(off, C.c_uint32),
                                             _ffi.ptr(adj, C.c_int32), flags, C.byref(h)))
         self._h = h
@@ -206,6 +210,19 @@ class ResidentGraph:
 
     def device_counts_ptr(self) -> int:
         return self._lib.acc_cn_device_counts(self._h) or 0
+
+    def csr(self):
+        """The resident CSR as a :class:`acctricnt_amd.graph.CSRGraph`, with
+        ``old_of_new`` the original id of every vertex (after a relabel at
+        upload; otherwise the ids set by :meth:`set_vertex_ids`)."""
+        from .graph import CSRGraph
+
+        off = np.empty(self.n + 1, np.uint32)
+        adj = np.empty(max(self.m, 1), np.int32)
+        old = np.empty(max(self.n, 1), np.int32)
+        check_cn(self._lib.acc_graph_download(self._h, _ffi.ptr(off, C.c_uint32), _ffi.ptr(adj, C.c_int32),
+                                              _ffi.ptr(old, C.c_int32)))
+        return CSRGraph(off, adj[:self.m].copy(), old_of_new=old[:self.n].copy(), name="resident")
 
     def set_vertex_ids(self, old_id) -> None:
         """Report cluster ids and memberships in these ids (old_id[new vertex])."""

@@ -12,6 +12,11 @@
 // Sets the thread's last error and returns code.
 int acc_set_error(int code, const std::string &msg);
 
+struct acc_graph;
+// Records a named HIP event on the graph's stream; acc_cn_timings reports the
+// intervals between consecutive events under the later event's name.
+int record_event(acc_graph *g, const char *name);
+
 #define ACC_HIP_TRY(expr)                                                                   \
   do {                                                                                      \
     hipError_t acc_e_ = (expr);                                                             \
@@ -77,6 +82,7 @@ struct acc_graph {
   int32_t *d_snbr = nullptr;   // neighbor of every sorted position
   uint32_t *d_sslot = nullptr; // original slot of every sorted position
   bool cnt_plus2 = true;                // counts stored as 2 + |N(u) ∩ N(v)|
+  bool relabelled = false;              // relabelled by degree at upload (d_old holds the old ids)
   // forward plan, binned by degree skew
   FwdUnit *d_fwd_wave = nullptr;  // wave packs of small sources: {b0, b1, upper slots, lower entries}
   uint32_t n_fwd_wave = 0;
@@ -230,4 +236,12 @@ int launch_combine(acc_graph *g, int plus);
 int launch_graph_totals(acc_graph *g, int plus);
 int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
                     hipStream_t s);
+// Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v})).
+int launch_vertex_meta(acc_graph *g, int32_t *lo);
 }  // namespace accfwd
+
+// Degree relabel on the device (k_relabel.hip): rewrites d_off and d_adj in
+// place (d_cnt is its scratch) and keeps the old id of every new vertex in d_old.
+namespace accrelabel {
+int relabel_degree(acc_graph *g);
+}  // namespace accrelabel

@@ -163,10 +163,6 @@ int upload_plans(acc_graph *g) {
     FwdUnit **d;
     uint32_t *count;
   };
-  if (!g->d_hs) {
-    ACC_TRY(dev_alloc(g, &g->d_hs, static_cast<uint64_t>(g->n) + 1));
-    ACC_HIP_TRY(hipMemcpyAsync(g->d_hs, g->h_hs.data(), sizeof(uint32_t) * g->n, hipMemcpyHostToDevice, g->ctx->stream));
-  }
   PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
                       {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
                       {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
@@ -204,7 +200,52 @@ int record(acc_graph *g, bool timing, const char *name) {
   return ACC_OK;
 }
 
+// Plan inputs of every vertex, computed on the device (k_vertex_meta): d_hs
+// stays resident, h_off, h_hs and h_lo are copied to the host for planning.
+// host_off, when given, is the caller's offsets array (equal to d_off).
+int load_vertex_meta(acc_graph *g, const uint32_t *host_off) {
+  const uint32_t n = g->n;
+  ACC_TRY(dev_alloc(g, &g->d_hs, static_cast<uint64_t>(n) + 1));
+  int32_t *lo = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&lo), sizeof(int32_t) * (n ? n : 1));
+  if (e != hipSuccess) return acc_set_error(ACC_ENOMEM, "hipMalloc of the vertex metadata failed");
+  g->h_off.resize(static_cast<size_t>(n) + 1);
+  g->h_hs.resize(n);
+  g->h_lo.resize(n);
+  int rc = accfwd::launch_vertex_meta(g, lo);
+  hipStream_t st = g->ctx->stream;
+  if (!rc) {
+    if (host_off)
+      std::copy(host_off, host_off + static_cast<size_t>(n) + 1, g->h_off.begin());
+    else
+      e = hipMemcpyAsync(g->h_off.data(), g->d_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
+                         hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && n)
+      e = hipMemcpyAsync(g->h_hs.data(), g->d_hs, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && n) e = hipMemcpyAsync(g->h_lo.data(), lo, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = acc_set_error(ACC_EDEVICE, std::string("vertex metadata: ") + hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(lo);
+  if (rc) return rc;
+  g->max_low = 0;
+  g->max_deg = 0;
+  for (uint32_t v = 0; v < n; v++) {
+    g->max_low = std::max(g->max_low, g->h_hs[v] - g->h_off[v]);
+    g->max_deg = std::max(g->max_deg, g->h_off[v + 1] - g->h_off[v]);
+  }
+  return ACC_OK;
+}
+
 }  // namespace
+
+int record_event(acc_graph *g, const char *name) {
+  if (g->events.empty()) g->timing_valid = false;
+  const int rc = record(g, true, name);
+  if (!rc && g->events.size() > 1) g->timing_valid = true;
+  return rc;
+}
 
 int acc_set_error(int code, const std::string &msg) {
   g_err = msg;
@@ -304,7 +345,8 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     e = hipMemcpyAsync(g->d_adj, edge_dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipMemsetAsync(g->d_flags, 0, sizeof(int32_t) * 4, ctx->stream);
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
-  if (!(flags & ACC_UPLOAD_NO_VALIDATE)) {
+  const bool relabel = (flags & ACC_UPLOAD_RELABEL) != 0;
+  if (!(flags & ACC_UPLOAD_NO_VALIDATE) || relabel) {  // the relabel indexes by neighbor id
     if ((rc = accfwd::launch_validate(g->d_off, g->d_adj, n, m, g->d_flags, ctx->stream))) return bail(rc);
     int32_t hf = 0;
     e = hipMemcpyAsync(&hf, g->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
@@ -315,18 +357,9 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
     if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
+  if (relabel && (rc = accrelabel::relabel_degree(g))) return bail(rc);
   if ((rc = accfwd::launch_build_rev(g))) return bail(rc);
-  g->h_off.assign(node_off, node_off + static_cast<size_t>(n) + 1);
-  g->h_hs.resize(n);
-  g->h_lo.resize(n);
-  for (uint32_t v = 0; v < n; v++) {
-    const int32_t *ub = std::upper_bound(edge_dst + node_off[v], edge_dst + node_off[v + 1], static_cast<int32_t>(v));
-    g->h_hs[v] = static_cast<uint32_t>(ub - edge_dst);
-    g->h_lo[v] = node_off[v + 1] > node_off[v] ? std::min(edge_dst[node_off[v]], static_cast<int32_t>(v))
-                                                 : static_cast<int32_t>(v);
-    g->max_low = std::max(g->max_low, g->h_hs[v] - node_off[v]);
-    g->max_deg = std::max(g->max_deg, node_off[v + 1] - node_off[v]);
-  }
+  if ((rc = load_vertex_meta(g, relabel ? nullptr : node_off))) return bail(rc);
   if ((rc = upload_plans(g))) return bail(rc);
   *out = g;
   return ACC_OK;
@@ -339,6 +372,21 @@ int acc_graph_info(const acc_graph *g, uint32_t *n, uint64_t *m, uint64_t *devic
   if (n) *n = g->n;
   if (m) *m = g->m;
   if (device_bytes) *device_bytes = g->device_bytes;
+  return ACC_OK;
+}
+
+int acc_graph_download(const acc_graph *g, uint32_t *node_off, int32_t *edge_dst, int32_t *old_id) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  DeviceGuard guard(g->ctx->device);
+  hipStream_t st = g->ctx->stream;
+  if (node_off) std::copy(g->h_off.begin(), g->h_off.end(), node_off);
+  if (edge_dst && g->m)
+    ACC_HIP_TRY(hipMemcpyAsync(edge_dst, g->d_adj, sizeof(int32_t) * g->m, hipMemcpyDeviceToHost, st));
+  if (old_id && g->d_old)
+    ACC_HIP_TRY(hipMemcpyAsync(old_id, g->d_old, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
+  else if (old_id)
+    for (uint32_t v = 0; v < g->n; v++) old_id[v] = static_cast<int32_t>(v);
+  ACC_HIP_TRY(hipStreamSynchronize(st));
   return ACC_OK;
 }
 

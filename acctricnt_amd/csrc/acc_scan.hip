@@ -938,7 +938,10 @@ int acc_graph_set_vertex_ids(acc_graph *g, const int32_t *old_id) {
   if (!g) return acc_set_error(ACC_EINVAL, "null graph");
   (void)hipSetDevice(g->ctx->device);
   if (!old_id) {
-    if (g->d_old) (void)hipFree(g->d_old);
+    if (g->d_old) {
+      (void)hipFree(g->d_old);
+      g->device_bytes -= sizeof(int32_t) * (g->n ? g->n : 1);
+    }
     g->d_old = nullptr;
     return ACC_OK;
   }

@@ -516,6 +516,8 @@ void acc_server_destroy(acc_server *s) { destroy_server(s); }
 int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
                          uint32_t flags, int32_t *graph_id) {
   if (!s || !node_off || (m && !edge_dst) || !graph_id) return acc_set_error(ACC_EINVAL, "null argument");
+  if (flags & ~ACC_UPLOAD_NO_VALIDATE)
+    return acc_set_error(ACC_EINVAL, "served graphs take only ACC_UPLOAD_NO_VALIDATE (relabel before adding)");
   if (node_off[0] != 0 || node_off[n] != m) return acc_set_error(ACC_EFORMAT, "node_off must run from 0 to m");
   std::unique_lock<std::mutex> lk(s->mu);
   const int32_t gid = s->next_gid++;

@@ -186,13 +186,16 @@ void SCAN_XP::Reorder() {
   old_vid_dict.clear();
   if (reorder_method == "deg") {
     log_info("Re-ordering... Method: DODG");
-    acc_hgraph *h = nullptr;
-    if (acc_hgraph_from_csr(g.nodemax, g.edgemax, g.node_off, g.edge_dst, &h) != ACC_OK)
-      die("re-ordering", acc_graph_last_error());
+    // On the device: the graph is uploaded, relabelled there and stays
+    // resident; the relabelled CSR and the old ids come back for the Graph.
+    check_cn(acc_ctx_create(0, &ctx_), "device");
+    check_cn(acc_graph_upload(ctx_, g.nodemax, g.edgemax, g.node_off, g.edge_dst, ACC_UPLOAD_RELABEL, &dg_),
+             "graph upload and relabel");
+    std::vector<uint32_t> noff(g.nodemax + 1ULL);
+    std::vector<int32_t> nadj(g.edgemax ? g.edgemax : 1);
     old_vid_dict.resize(g.nodemax);
-    if (acc_hgraph_relabel_degree(h, old_vid_dict.data()) != ACC_OK) die("re-ordering", acc_graph_last_error());
-    g.ReplaceCSR(acc_hgraph_offsets(h), acc_hgraph_adj(h));
-    acc_hgraph_free(h);
+    check_cn(acc_graph_download(dg_, noff.data(), nadj.data(), old_vid_dict.data()), "relabelled graph");
+    g.ReplaceCSR(noff.data(), nadj.data());
   } else if (reorder_method == "gro" || reorder_method == "cache" || reorder_method == "bfsr" ||
              reorder_method == "dfs" || reorder_method == "hybrid" || reorder_method == "rcm-cache" ||
              reorder_method == "slashburn") {
@@ -233,6 +236,10 @@ void SCAN_XP::Reorder() {
 
 void SCAN_XP::Upload() {
   auto t0 = high_resolution_clock::now();
+  if (dg_) {  // uploaded and relabelled by Reorder; cluster ids already in old ids
+    log_info("GPU upload, validation and relabel: done in the re-ordering step");
+    return;
+  }
   check_cn(acc_ctx_create(0, &ctx_), "device");
   check_cn(acc_graph_upload(ctx_, g.nodemax, g.edgemax, g.node_off, g.edge_dst, 0, &dg_), "graph upload");
   if (!old_vid_dict.empty()) check_cn(acc_graph_set_vertex_ids(dg_, old_vid_dict.data()), "vertex ids");

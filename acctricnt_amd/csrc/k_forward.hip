@@ -56,6 +56,23 @@ __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_
   if (tid == 0) *sum = 0;
 }
 
+// Plan inputs of every vertex v: hs[v], the first slot of N(v) holding an id
+// above v (the start of its upper slots), and lo[v] = min(N(v) ∪ {v}).
+__global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict__ off,
+                                                     const int32_t *__restrict__ adj, uint32_t n,
+                                                     uint32_t *__restrict__ hs, int32_t *__restrict__ lo) {
+  for (uint32_t v = blockIdx.x * 256u + threadIdx.x; v < n; v += gridDim.x * 256u) {
+    const uint32_t s = off[v], e = off[v + 1];
+    uint32_t a = s, b = e;
+    while (a < b) {
+      const uint32_t mid = a + ((b - a) >> 1);
+      if (adj[mid] <= static_cast<int32_t>(v)) a = mid + 1; else b = mid;
+    }
+    hs[v] = a;
+    lo[v] = e > s ? min(adj[s], static_cast<int32_t>(v)) : static_cast<int32_t>(v);
+  }
+}
+
 // Reverse slots, computed once per resident graph: rev[j] = position of u in
 // N(v) for every upper slot j = (u, v), u < v (binary search of N(v)), ~0
 // for lower slots and for an upper slot whose reverse is absent (possible
@@ -725,6 +742,14 @@ int launch_build_rev(acc_graph *g) {
   const uint64_t cap = static_cast<uint64_t>(g->ctx->num_cus > 0 ? g->ctx->num_cus : 256) * 64;
   hipLaunchKernelGGL(k_build_rev, dim3(static_cast<unsigned>(want < cap ? want : cap)), dim3(256), 0,
                      g->ctx->stream, g->d_off, g->d_adj, g->n, g->d_rev);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_vertex_meta(acc_graph *g, int32_t *lo) {
+  if (!g->n) return ACC_OK;
+  hipLaunchKernelGGL(k_vertex_meta, dim3(grid_for(g->n, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_off,
+                     g->d_adj, g->n, g->d_hs, lo);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }

@@ -1,0 +1,166 @@
+// Degree relabel of a resident CSR on the device: new ids by degree,
+// descending, ties by ascending old id, every adjacency list mapped and
+// sorted ascending.  This is the deterministic form of the reference's
+// ReorderDegDescending + Reorder (scan-query/scan_xp.cpp:1279-1301 builds
+// the order with an unstable parallel sort, :1145-1222 rebuilds and re-sorts
+// the adjacency lists); the old id of every new vertex is kept on the graph
+// (the reference's old_vid_dict) so cluster ids come out in old ids.
+//
+// Steps, all HBM-bound: degrees (n), a stable radix sort of (degree, id)
+// pairs by degree descending (n), the inverse permutation and the new
+// offsets (an exclusive scan of the sorted degrees), one wave per old vertex
+// writing its mapped neighbors at the new vertex's offset (m), and a
+// segmented radix sort of the new lists over ceil(log2 n) key bits (m).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <string>
+
+#include "acc_internal.h"
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_rl_keys(const uint32_t *__restrict__ off, uint32_t n,
+                                                 uint32_t *__restrict__ deg, int32_t *__restrict__ id) {
+  for (uint32_t v = blockIdx.x * 256u + threadIdx.x; v < n; v += gridDim.x * 256u) {
+    deg[v] = off[v + 1] - off[v];
+    id[v] = static_cast<int32_t>(v);
+  }
+}
+
+// x is a new id; deg_sorted[x] is its degree.  len gets n + 1 entries, the
+// last 0, so that its exclusive scan ends with m.
+__global__ __launch_bounds__(256) void k_rl_inverse(const uint32_t *__restrict__ deg_sorted,
+                                                    const int32_t *__restrict__ old_of_new, uint32_t n,
+                                                    int32_t *__restrict__ new_of_old, uint32_t *__restrict__ len) {
+  for (uint32_t x = blockIdx.x * 256u + threadIdx.x; x <= n; x += gridDim.x * 256u) {
+    if (x == n) {
+      len[n] = 0;
+    } else {
+      new_of_old[old_of_new[x]] = static_cast<int32_t>(x);
+      len[x] = deg_sorted[x];
+    }
+  }
+}
+
+// One wave per old vertex o: its neighbors, mapped to new ids, go to the new
+// vertex's range [new_off[x], new_off[x] + deg) in old order.
+__global__ __launch_bounds__(256) void k_rl_fill(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                 uint32_t n, const int32_t *__restrict__ new_of_old,
+                                                 const uint32_t *__restrict__ new_off, uint32_t *__restrict__ out) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * 256u + threadIdx.x) >> 6);
+  const uint32_t nwaves = gridDim.x * 4u;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t o = wave; o < n; o += nwaves) {
+    const uint32_t b = off[o], e = off[o + 1];
+    if (b == e) continue;
+    const uint32_t w = new_off[new_of_old[o]];
+    for (uint32_t i = b + lane; i < e; i += 64) out[w + (i - b)] = static_cast<uint32_t>(new_of_old[adj[i]]);
+  }
+}
+
+unsigned rl_grid(uint64_t items, int cus) {
+  const uint64_t want = (items + 255) / 256;
+  const uint64_t cap = static_cast<uint64_t>(cus > 0 ? cus : 256) * 8;
+  return static_cast<unsigned>(want < cap ? (want ? want : 1) : cap);
+}
+
+int bit_width(uint32_t x) { return x ? 32 - __builtin_clz(x) : 1; }
+
+}  // namespace
+
+namespace accrelabel {
+
+int relabel_degree(acc_graph *g) {
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  if (n == 0) return ACC_OK;
+  if (m >= (1ULL << 31)) return acc_set_error(ACC_EINVAL, "the device relabel needs m < 2^31");
+  hipStream_t st = g->ctx->stream;
+  const int cus = g->ctx->num_cus;
+  const int bits = bit_width(n - 1);  // ids and degrees are below n
+  uint32_t *deg = nullptr, *deg_sorted = nullptr, *len = nullptr, *new_off = nullptr;
+  int32_t *id = nullptr, *old_of_new = nullptr, *new_of_old = nullptr;
+  void *temp = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (void *p : {static_cast<void *>(deg), static_cast<void *>(deg_sorted), static_cast<void *>(len),
+                    static_cast<void *>(new_off), static_cast<void *>(id), static_cast<void *>(new_of_old), temp})
+      if (p) (void)hipFree(p);
+  };
+  auto fail = [&](const char *what, hipError_t e) {
+    cleanup();
+    if (old_of_new) (void)hipFree(old_of_new);
+    return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
+                         std::string("relabel ") + what + ": " + hipGetErrorString(e));
+  };
+  const size_t nb = sizeof(uint32_t) * n, nb1 = sizeof(uint32_t) * (static_cast<size_t>(n) + 1);
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&deg), nb);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&deg_sorted), nb);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&id), nb);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&old_of_new), nb);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&new_of_old), nb);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&len), nb1);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&new_off), nb1);
+  if (e != hipSuccess) return fail("allocation", e);
+  // cub temp storage: the largest of the three primitives' needs
+  size_t t_sort = 0, t_scan = 0, t_seg = 0;
+  uint32_t *mapped = reinterpret_cast<uint32_t *>(g->d_cnt);  // the count array is free until the first pass
+  uint32_t *lists = reinterpret_cast<uint32_t *>(g->d_adj);
+  e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t_sort, deg, deg_sorted, id, old_of_new,
+                                                   static_cast<int>(n), 0, bits, st);
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, len, new_off, static_cast<int>(n) + 1, st);
+  if (e == hipSuccess && m)
+    e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, t_seg, mapped, lists, static_cast<int>(m),
+                                                   static_cast<int>(n), new_off, new_off + 1, 0, bits, st);
+  size_t t_bytes = std::max(t_sort, std::max(t_scan, t_seg));
+  if (e == hipSuccess) e = hipMalloc(&temp, t_bytes ? t_bytes : 1);
+  if (e != hipSuccess) return fail("temporary storage", e);
+
+  // device time of each step, read back with acc_cn_timings
+  for (hipEvent_t ev : g->events) (void)hipEventDestroy(ev);
+  g->events.clear();
+  g->event_names.clear();
+  g->timing_valid = false;
+  int rc = record_event(g, "begin");
+  if (rc) {
+    cleanup();
+    (void)hipFree(old_of_new);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_rl_keys, dim3(rl_grid(n, cus)), dim3(256), 0, st, g->d_off, n, deg, id);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairsDescending(temp, t_bytes, deg, deg_sorted, id, old_of_new,
+                                                     static_cast<int>(n), 0, bits, st);
+  if (e != hipSuccess) return fail("degree order", e);
+  (void)record_event(g, "rl_degree_order");
+  hipLaunchKernelGGL(k_rl_inverse, dim3(rl_grid(static_cast<uint64_t>(n) + 1, cus)), dim3(256), 0, st, deg_sorted,
+                     old_of_new, n, new_of_old, len);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(temp, t_bytes, len, new_off, static_cast<int>(n) + 1, st);
+  if (e != hipSuccess) return fail("offsets", e);
+  (void)record_event(g, "rl_offsets");
+  if (m) {
+    hipLaunchKernelGGL(k_rl_fill, dim3(rl_grid(static_cast<uint64_t>(n) * 64, cus)), dim3(256), 0, st, g->d_off,
+                       g->d_adj, n, new_of_old, new_off, mapped);
+    e = hipGetLastError();
+    if (e == hipSuccess) (void)record_event(g, "rl_fill");
+    if (e == hipSuccess)
+      e = hipcub::DeviceSegmentedRadixSort::SortKeys(temp, t_bytes, mapped, lists, static_cast<int>(m),
+                                                     static_cast<int>(n), new_off, new_off + 1, 0, bits, st);
+    if (e != hipSuccess) return fail("adjacency lists", e);
+    (void)record_event(g, "rl_sort_lists");
+  }
+  e = hipMemcpyAsync(g->d_off, new_off, nb1, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return fail("offsets copy", e);
+  (void)record_event(g, "rl_copy_offsets");
+  cleanup();
+  if (g->d_old) (void)hipFree(g->d_old);
+  else g->device_bytes += nb;
+  g->d_old = old_of_new;
+  g->relabelled = true;
+  return ACC_OK;
+}
+
+}  // namespace accrelabel

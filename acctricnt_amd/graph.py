@@ -105,6 +105,13 @@ def from_edges(n: int, src, dst, relabel: bool = False, name: str = "edges") -> 
     return _finish(h, relabel, name, {"kind": "edges"})
 
 
+def relabel_degree(g: CSRGraph) -> CSRGraph:
+    """A copy of g relabelled on the host by degree, descending, ties by
+    ascending id (``acc_hgraph_relabel_degree``); ``old_of_new`` holds the
+    original id of every new vertex."""
+    return _finish(_to_handle(g), True, g.name, dict(g.meta))
+
+
 def from_csr(node_off, edge_dst, name: str = "csr") -> CSRGraph:
     """Validate caller CSR arrays (sorted, symmetric, no self-loops) and wrap them."""
     g = CSRGraph(np.ascontiguousarray(node_off, dtype=np.uint32),

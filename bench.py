@@ -114,6 +114,49 @@ def calibration_copy(mib: int) -> None:
     print(f"[bench] calibration copy of {mib} MiB done", file=sys.stderr, flush=True)
 
 
+def bench_relabel(ctx, spec, relabelled, reps: int = 3):
+    """The degree relabel of the workload graph as generated (ids not
+    relabelled): on the device (HIP events around each relabel step at
+    upload, best of reps; the uploads' wall clock with and without the
+    relabel, PCIe included, beside it) and on the host (the graph library's
+    parallel relabel), checked equal to the relabelled graph the bench
+    counts on."""
+    import numpy as np
+
+    from acctricnt_amd import graph as G
+
+    if spec["kind"] != "rmat":
+        return None
+    raw = G.gen_rmat(spec["scale"], spec["edge_factor"], seed=spec["seed"], relabel=False)
+    t_plain, t_rel, dev, same = [], [], [], True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rg = ctx.upload(raw)
+        ctx.synchronize()
+        t_plain.append(time.perf_counter() - t0)
+        rg.release()
+        t0 = time.perf_counter()
+        rg = ctx.upload(raw, relabel=True)
+        ctx.synchronize()
+        t_rel.append(time.perf_counter() - t0)
+        steps = rg.timings()
+        dev.append((sum(steps.values()), steps))
+        r = rg.csr()
+        rg.release()
+        same = same and bool(np.array_equal(r.node_off, relabelled.node_off)
+                             and np.array_equal(r.edge_dst, relabelled.edge_dst)
+                             and np.array_equal(r.old_of_new, relabelled.old_of_new))
+    t0 = time.perf_counter()
+    G.relabel_degree(raw)
+    t_host = time.perf_counter() - t0
+    best = min(dev, key=lambda x: x[0])
+    res = {"device_relabel_ms": best[0], "device_steps_ms": best[1], "host_relabel_ms": t_host * 1e3,
+           "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())), "upload_ms": min(t_plain) * 1e3,
+           "upload_relabel_ms": min(t_rel) * 1e3, "identical_to_host_relabel": same}
+    print(f"[relabel] {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def bench_batch(ctx, steps: int, warmup: int, num_graphs: int = 512):
     """Batched form on the 512-graph mix: one packed multi-graph CSR, autotuned,
     its counting pass captured in a HIP graph and replayed with one launch per
@@ -317,6 +360,7 @@ def main():
                     help="tuning parameter of the resident graph (repeatable)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain measurement")
+    ap.add_argument("--no-relabel", action="store_true", help="skip the device-relabel measurement")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
     ap.add_argument("--serve-queries", type=int, default=6000)
@@ -389,6 +433,7 @@ def main():
     rev = g.reverse_slots()
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
     rg.release()
+    relabel = None if args.no_relabel else bench_relabel(ctx, spec, g)
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
     served = None if args.no_serve else bench_serve(
         ctx, args.serve_queries, args.serve_rate, streams=args.serve_streams,
@@ -421,6 +466,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, args.workload),
                      "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
+        "relabel": relabel,
         "batch": batch,
         "serve": served,
     }

@@ -49,6 +49,10 @@ typedef struct acc_graph acc_graph;
 
 /* Upload flags. */
 #define ACC_UPLOAD_NO_VALIDATE 0x1u /* skip the host CSR validation (trusted input) */
+#define ACC_UPLOAD_RELABEL 0x2u     /* relabel on the device by degree, descending, ties by
+                                       ascending id; implies validation.  See acc_graph_download;
+                                       acc_cn_timings then gives the relabel's device time per step
+                                       until the next timed pass. */
 
 /* Context -------------------------------------------------------------- */
 int acc_device_count(int *count);
@@ -68,6 +72,16 @@ const char *acc_last_error(void);
 int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_off,
                      const int32_t *edge_dst, uint32_t flags, acc_graph **out);
 void acc_graph_release(acc_graph *g);
+/* The resident CSR and the caller id of every vertex (any pointer may be NULL).
+ * After an ACC_UPLOAD_RELABEL upload this is the relabelled graph and
+ * old_id[new vertex] = original id: what SCAN_XP::Reorder with the "deg"
+ * method leaves in the Graph and in old_vid_dict (scan-query/scan_xp.cpp:1145-1222,
+ * 1279-1301; the reference's order among equal degrees is unspecified, here it is
+ * ascending original id), and the graph reports cluster ids in original ids.
+ * Otherwise the uploaded CSR and the ids set by acc_graph_set_vertex_ids
+ * (the identity by default). */
+int acc_graph_download(const acc_graph *g, uint32_t *node_off /* n+1 */, int32_t *edge_dst /* m */,
+                       int32_t *old_id /* n */);
 /* n, m and the device bytes the graph holds (CSR, counts, scratch, plans). */
 int acc_graph_info(const acc_graph *g, uint32_t *n, uint64_t *m, uint64_t *device_bytes);
 
@@ -212,7 +226,8 @@ void acc_server_config_default(acc_server_config *cfg); /* 64 GiB, batch 14, 4 s
 int acc_server_create(acc_ctx *ctx, const acc_server_config *cfg, acc_server **out);
 void acc_server_destroy(acc_server *s);
 /* Registers a CSR (copied) and admits it; the first admission validates it
- * unless flags has ACC_UPLOAD_NO_VALIDATE.  ACC_EBUDGET if it cannot fit.
+ * unless flags has ACC_UPLOAD_NO_VALIDATE (the only flag taken here; ACC_EINVAL
+ * for others).  ACC_EBUDGET if it cannot fit.
  * Idle result slots are grown for the graph here, within the budget, so
  * that serving queries allocates nothing. */
 int acc_server_add_graph(acc_server *s, uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,

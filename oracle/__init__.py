@@ -171,6 +171,30 @@ def write_result(path: str, g, res, old_id=None) -> None:
         raise OSError(f"cannot write {path}")
 
 
+def relabel_degree(g):
+    """Degree relabel as (node_off, edge_dst, old_of_new): new ids by degree,
+    descending, ties by ascending old id, every list mapped and sorted.
+
+    Follows SCAN_XP::Reorder (scan-query/scan_xp.cpp:1145-1222: old_vid_dict,
+    then each list mapped through the new ids and re-sorted) with the order of
+    ReorderDegDescending (:1279-1301), whose unstable parallel sort leaves
+    equal degrees in an unspecified order; here they keep ascending old id.
+    Written in numpy (a stable argsort and one sort of (source, neighbor)
+    keys); pinned in tests/test_oracle.py by the properties that determine
+    the result uniquely."""
+    n = g.n
+    deg = np.diff(np.asarray(g.node_off, dtype=np.int64))
+    old_of_new = np.argsort(-deg, kind="stable").astype(np.int32)
+    new_of_old = np.empty(n, dtype=np.int64)
+    new_of_old[old_of_new] = np.arange(n, dtype=np.int64)
+    off = np.zeros(n + 1, dtype=np.uint32)
+    np.cumsum(deg[old_of_new], out=off[1:])
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    keys = (new_of_old[src] << 32) | new_of_old[np.asarray(g.edge_dst, dtype=np.int64)]
+    keys.sort()
+    return off, (keys & 0xFFFFFFFF).astype(np.int32), old_of_new
+
+
 def threads() -> int:
     return int(lib().oracle_threads())
 

@@ -200,6 +200,73 @@ def test_planted_mistake_is_caught():
     assert not np.array_equal(bad, bad[rev])
 
 
+# ------------------------------------------------------------ degree relabel
+def relabel_properties_hold(g, off, adj, old) -> bool:
+    """The properties that determine a degree relabel uniquely: old is a
+    permutation, new degrees are old degrees through it, non-increasing, equal
+    degrees in ascending old id, every list strictly increasing, and the edge
+    set maps back onto the original one."""
+    n = g.n
+    if sorted(old.tolist()) != list(range(n)) or off.shape[0] != n + 1 or adj.shape[0] != g.m:
+        return False
+    d_old, d_new = g.degrees(), np.diff(off.astype(np.int64))
+    if not np.array_equal(d_old[old], d_new) or (np.diff(d_new) > 0).any():
+        return False
+    if ((np.diff(d_new) == 0) & (np.diff(old.astype(np.int64)) < 0)).any():
+        return False
+    src = np.repeat(np.arange(n), d_new)
+    same_src = src[1:] == src[:-1]
+    if (np.diff(adj.astype(np.int64))[same_src] <= 0).any():
+        return False
+    mapped = set(zip(old[src].tolist(), old[adj].tolist()))
+    return mapped == set(zip(g.sources().tolist(), g.edge_dst.tolist()))
+
+
+RELABEL_GRAPHS = [
+    lambda: G.gen_rmat(10, 16, seed=71),
+    lambda: G.gen_er(3000, 6, seed=72),
+    lambda: G.gen_chung_lu(4000, 2.1, 10, 600, seed=73),
+    lambda: G.from_edges(9, [0, 0, 0, 0, 5, 6], [1, 2, 3, 4, 6, 7]),  # star, path, isolated 8
+    lambda: G.from_edges(6, [0, 1, 2, 3, 4, 5], [1, 2, 3, 4, 5, 0]),  # cycle: every degree equal
+    lambda: G.from_edges(4, [], []),
+]
+
+
+@pytest.mark.parametrize("maker", RELABEL_GRAPHS)
+def test_relabel_restatement_pinned(maker):
+    g = maker()
+    off, adj, old = oracle.relabel_degree(g)
+    assert relabel_properties_hold(g, off, adj, old)
+    # brute force from the definition on the small ones
+    if g.n <= 16:
+        d = g.degrees()
+        order = sorted(range(g.n), key=lambda v: (-int(d[v]), v))
+        new = {v: i for i, v in enumerate(order)}
+        lists = [sorted(new[int(w)] for w in g.edge_dst[g.node_off[v]:g.node_off[v + 1]]) for v in order]
+        assert old.tolist() == order
+        assert adj.tolist() == [w for lst in lists for w in lst]
+    # the product's host relabel (libacc_graph) agrees
+    h = G.relabel_degree(g)
+    assert np.array_equal(h.node_off, off) and np.array_equal(h.edge_dst, adj)
+    assert np.array_equal(h.old_of_new, old)
+
+
+def test_relabel_planted_mistakes_are_caught():
+    """Swapping two equal-degree vertices, or leaving one list unsorted, breaks the properties."""
+    g = G.gen_rmat(9, 8, seed=74)
+    off, adj, old = oracle.relabel_degree(g)
+    d = np.diff(off.astype(np.int64))
+    i = int(np.nonzero(np.diff(d) == 0)[0][0])
+    bad_old = old.copy()
+    bad_old[[i, i + 1]] = bad_old[[i + 1, i]]
+    assert not relabel_properties_hold(g, off, adj, bad_old)
+    v = int(np.nonzero(d >= 2)[0][0])
+    bad_adj = adj.copy()
+    a, b = int(off[v]), int(off[v])
+    bad_adj[[a, b + 1]] = bad_adj[[b + 1, a]]
+    assert not relabel_properties_hold(g, off, bad_adj, old)
+
+
 # ------------------------------------------------------------ SCAN semantics
 @pytest.mark.parametrize("eps,mu", [("0.2", 5), ("0.5", 3), ("0.7", 2), ("0.35", 4)])
 def test_scan_vs_bruteforce(eps, mu):
