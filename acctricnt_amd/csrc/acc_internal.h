@@ -227,21 +227,29 @@ int launch_index_jobs(const JobTable &jt, uint32_t vblocks, uint32_t sblocks, hi
 // Kernel launchers (k_forward.hip).
 namespace accfwd {
 int launch_init(acc_graph *g);
-int launch_build_rev(acc_graph *g);
+// Slot-parallel over the sources in d_cnt (accprep::slot_sources).  With
+// validate, d_flags[0] gets flag 8 for an upper slot without reverse and
+// d_flags[2..3] (a uint64) the number of upper slots.
+int launch_build_rev(acc_graph *g, bool validate);
 // Triangle total only (forward path, no per-slot counts) into d_sum[1].
 int launch_triangles(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_count_big(acc_graph *g);
 int launch_combine(acc_graph *g, int plus);
 int launch_graph_totals(acc_graph *g, int plus);
-int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
-                    hipStream_t s);
+// Slot-parallel over the sources in d_cnt (accprep::slot_sources).
+int launch_validate(acc_graph *g);
 // Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v})).
 int launch_vertex_meta(acc_graph *g, int32_t *lo);
 }  // namespace accfwd
 
-// Degree relabel on the device (k_relabel.hip): rewrites d_off and d_adj in
-// place (d_cnt is its scratch) and keeps the old id of every new vertex in d_old.
-namespace accrelabel {
+// Upload-time preparation (k_prepare.hip).  slot_sources writes the source
+// vertex of every slot into d_cnt (d_rev is its scratch); both arrays are
+// free until the reverse slots are built and the first pass runs.
+// relabel_degree, given those sources, rewrites d_off and d_adj in place by
+// degree (d_rev is its scratch) and keeps the old id of every new vertex in
+// d_old; the sources must be recomputed after it.
+namespace accprep {
+int slot_sources(acc_graph *g);
 int relabel_degree(acc_graph *g);
-}  // namespace accrelabel
+}  // namespace accprep

@@ -346,19 +346,29 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
   if (e == hipSuccess) e = hipMemsetAsync(g->d_flags, 0, sizeof(int32_t) * 4, ctx->stream);
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
   const bool relabel = (flags & ACC_UPLOAD_RELABEL) != 0;
-  if (!(flags & ACC_UPLOAD_NO_VALIDATE) || relabel) {  // the relabel indexes by neighbor id
-    if ((rc = accfwd::launch_validate(g->d_off, g->d_adj, n, m, g->d_flags, ctx->stream))) return bail(rc);
-    int32_t hf = 0;
-    e = hipMemcpyAsync(&hf, g->d_flags, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("validate: ") + hipGetErrorString(e)));
-    if (hf & 1) return bail(acc_set_error(ACC_EFORMAT, "neighbor id out of range"));
-    if (hf & 2) return bail(acc_set_error(ACC_EFORMAT, "Self loop"));
-    if (hf & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
-    if (hf & 8) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
+  const bool validate = !(flags & ACC_UPLOAD_NO_VALIDATE) || relabel;  // the relabel indexes by neighbor id
+  int32_t hf[4] = {0, 0, 0, 0};
+  auto read_flags = [&]() {
+    hipError_t err = hipMemcpyAsync(hf, g->d_flags, sizeof(hf), hipMemcpyDeviceToHost, ctx->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(ctx->stream);
+    return err == hipSuccess ? ACC_OK
+                             : acc_set_error(ACC_EDEVICE, std::string("validate: ") + hipGetErrorString(err));
+  };
+  if ((rc = accprep::slot_sources(g))) return bail(rc);
+  if (validate) {
+    if ((rc = accfwd::launch_validate(g)) || (rc = read_flags())) return bail(rc);
+    if (hf[0] & 1) return bail(acc_set_error(ACC_EFORMAT, "neighbor id out of range"));
+    if (hf[0] & 2) return bail(acc_set_error(ACC_EFORMAT, "Self loop"));
+    if (hf[0] & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
   }
-  if (relabel && (rc = accrelabel::relabel_degree(g))) return bail(rc);
-  if ((rc = accfwd::launch_build_rev(g))) return bail(rc);
+  if (relabel && ((rc = accprep::relabel_degree(g)) || (rc = accprep::slot_sources(g)))) return bail(rc);
+  if ((rc = accfwd::launch_build_rev(g, validate))) return bail(rc);
+  if (validate) {
+    if ((rc = read_flags())) return bail(rc);
+    uint64_t upper = 0;
+    std::memcpy(&upper, hf + 2, sizeof(upper));
+    if ((hf[0] & 8) || 2 * upper != m) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
+  }
   if ((rc = load_vertex_meta(g, relabel ? nullptr : node_off))) return bail(rc);
   if ((rc = upload_plans(g))) return bail(rc);
   *out = g;

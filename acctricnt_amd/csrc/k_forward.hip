@@ -76,32 +76,44 @@ __global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict_
 // Reverse slots, computed once per resident graph: rev[j] = position of u in
 // N(v) for every upper slot j = (u, v), u < v (binary search of N(v)), ~0
 // for lower slots and for an upper slot whose reverse is absent (possible
-// only in an unvalidated, asymmetric CSR).  One wave per vertex.
+// only in an unvalidated, asymmetric CSR).  One thread per slot, the source
+// u of slot j read from src.  With flags (validation): flag 8 for an upper
+// slot without reverse, and the number of upper slots added to *upper.  In a
+// CSR with sorted, duplicate-free lists and no self-loop, every upper slot
+// having a reverse maps the upper slots one-to-one into the lower ones, so
+// the CSR is symmetric iff additionally 2 * upper == m.
 __global__ __launch_bounds__(256) void k_build_rev(const uint32_t *__restrict__ off,
-                                                   const int32_t *__restrict__ adj, uint32_t n,
-                                                   uint32_t *__restrict__ rev) {
-  const uint32_t waves = gridDim.x * 4u;
-  const int lane = threadIdx.x & 63;
-  for (uint32_t v = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6)); v < n; v += waves) {
-    const uint32_t s = off[v], e = off[v + 1];
-    for (uint32_t j = s + lane; j < e; j += 64) {
-      const int32_t c = adj[j];
-      uint32_t p = ~0u;
-      if (c > static_cast<int32_t>(v)) {
-        uint32_t lo = off[c];
-        const uint32_t end = off[c + 1];
-        uint32_t hi = end;
-        while (lo < hi) {
-          const uint32_t mid = lo + (hi - lo) / 2;
-          if (adj[mid] < static_cast<int32_t>(v))
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        if (lo < end && adj[lo] == static_cast<int32_t>(v)) p = lo;
+                                                   const int32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ src, uint32_t n, uint64_t m,
+                                                   uint32_t *__restrict__ rev, int32_t *__restrict__ flags,
+                                                   unsigned long long *__restrict__ upper) {
+  int32_t f = 0;
+  unsigned long long up = 0;
+  for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < m; j += gridDim.x * 256ull) {
+    const int32_t v = static_cast<int32_t>(src[j]);
+    const int32_t c = adj[j];
+    uint32_t p = ~0u;
+    if (c > v && static_cast<uint32_t>(c) < n) {
+      uint32_t lo = off[c];
+      const uint32_t end = off[c + 1];
+      uint32_t hi = end;
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (adj[mid] < v)
+          lo = mid + 1;
+        else
+          hi = mid;
       }
-      rev[j] = p;
+      if (lo < end && adj[lo] == v) p = lo;
+      up++;
+      if (p == ~0u) f |= 8;
     }
+    rev[j] = p;
+  }
+  if (flags) {
+    up = wave_sum_u64(up);
+    if (f) atomicOr(flags, f);
+    if ((threadIdx.x & 63) == 0 && up) atomicAdd(upper, up);
   }
 }
 
@@ -683,22 +695,18 @@ __global__ __launch_bounds__(256) void k_graph_totals(const uint32_t *__restrict
   }
 }
 
-// Flags: 1 id out of range, 2 self-loop, 4 unsorted, 8 asymmetric.
+// Flags: 1 id out of range, 2 self-loop, 4 unsorted (duplicates included).
+// Symmetry is checked by k_build_rev.  The source of slot i is src[i]
+// (off[src[i]] <= i < off[src[i] + 1]).
 __global__ __launch_bounds__(256) void k_validate(const uint32_t *__restrict__ off,
-                                                  const int32_t *__restrict__ adj, uint32_t n, uint64_t m,
+                                                  const int32_t *__restrict__ adj,
+                                                  const uint32_t *__restrict__ src, uint32_t n, uint64_t m,
                                                   int32_t *__restrict__ flags) {
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
   const uint64_t stride = gridDim.x * 256ull;
   int32_t f = 0;
   for (uint64_t i = tid; i < m; i += stride) {
-    uint32_t lo = 0, hi = n;  // off[lo] <= i < off[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = lo + ((hi - lo) >> 1);
-      if (off[mid] <= i)
-        lo = mid;
-      else
-        hi = mid;
-    }
+    const uint32_t lo = src[i];
     const int32_t v = adj[i];
     if (v < 0 || static_cast<uint32_t>(v) >= n) {
       f |= 1;
@@ -706,15 +714,6 @@ __global__ __launch_bounds__(256) void k_validate(const uint32_t *__restrict__ o
     }
     if (static_cast<uint32_t>(v) == lo) f |= 2;
     if (i > off[lo] && adj[i - 1] >= v) f |= 4;
-    uint32_t a = off[v], e = off[v + 1];
-    while (a < e) {
-      const uint32_t mid = a + ((e - a) >> 1);
-      if (adj[mid] < static_cast<int32_t>(lo))
-        a = mid + 1;
-      else
-        e = mid;
-    }
-    if (a >= off[v + 1] || adj[a] != static_cast<int32_t>(lo)) f |= 8;
   }
   if (f) atomicOr(flags, f);
 }
@@ -736,12 +735,12 @@ int launch_init(acc_graph *g) {
   return ACC_OK;
 }
 
-int launch_build_rev(acc_graph *g) {
-  if (!g->n) return ACC_OK;
-  const uint64_t want = (static_cast<uint64_t>(g->n) + 3) / 4;
-  const uint64_t cap = static_cast<uint64_t>(g->ctx->num_cus > 0 ? g->ctx->num_cus : 256) * 64;
-  hipLaunchKernelGGL(k_build_rev, dim3(static_cast<unsigned>(want < cap ? want : cap)), dim3(256), 0,
-                     g->ctx->stream, g->d_off, g->d_adj, g->n, g->d_rev);
+int launch_build_rev(acc_graph *g, bool validate) {
+  if (!g->m) return ACC_OK;
+  hipLaunchKernelGGL(k_build_rev, dim3(grid_for(g->m, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_off,
+                     g->d_adj, reinterpret_cast<const uint32_t *>(g->d_cnt), g->n, g->m, g->d_rev,
+                     validate ? g->d_flags : nullptr,
+                     reinterpret_cast<unsigned long long *>(g->d_flags + 2));
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
@@ -834,9 +833,10 @@ int launch_graph_totals(acc_graph *g, int plus) {
   return ACC_OK;
 }
 
-int launch_validate(const uint32_t *d_off, const int32_t *d_adj, uint32_t n, uint64_t m, int32_t *d_flags,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(k_validate, dim3(grid_for(m, 256)), dim3(256), 0, s, d_off, d_adj, n, m, d_flags);
+int launch_validate(acc_graph *g) {
+  if (!g->m) return ACC_OK;
+  hipLaunchKernelGGL(k_validate, dim3(grid_for(g->m, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_off,
+                     g->d_adj, reinterpret_cast<const uint32_t *>(g->d_cnt), g->n, g->m, g->d_flags);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }

@@ -1,3 +1,11 @@
+// Upload-time preparation of a resident CSR on the device.
+//
+// Slot sources: every non-empty vertex o marks its first slot with o in a
+// zeroed array and an inclusive max-scan spreads the marks, so slot j gets
+// the vertex whose range holds it.  The work is per slot, not per vertex, so
+// a hub costs no more than its slots; validation, the reverse-slot build and
+// the relabel's fill are slot-parallel on top of it.
+//
 // Degree relabel of a resident CSR on the device: new ids by degree,
 // descending, ties by ascending old id, every adjacency list mapped and
 // sorted ascending.  This is the deterministic form of the reference's
@@ -8,9 +16,9 @@
 //
 // Steps, all HBM-bound: degrees (n), a stable radix sort of (degree, id)
 // pairs by degree descending (n), the inverse permutation and the new
-// offsets (an exclusive scan of the sorted degrees), one wave per old vertex
-// writing its mapped neighbors at the new vertex's offset (m), and a
-// segmented radix sort of the new lists over ceil(log2 n) key bits (m).
+// offsets (an exclusive scan of the sorted degrees), one thread per old slot
+// writing its neighbor's new id at the new position (m), and a segmented
+// radix sort of the new lists over ceil(log2 n) key bits (m).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -43,19 +51,26 @@ __global__ __launch_bounds__(256) void k_rl_inverse(const uint32_t *__restrict__
   }
 }
 
-// One wave per old vertex o: its neighbors, mapped to new ids, go to the new
-// vertex's range [new_off[x], new_off[x] + deg) in old order.
+// Every non-empty vertex o marks its first slot with o (heads is zeroed
+// first); an inclusive max-scan then gives the source of every slot.  Needs
+// only monotone offsets ending at m.
+__global__ __launch_bounds__(256) void k_rl_heads(const uint32_t *__restrict__ off, uint32_t n,
+                                                  uint32_t *__restrict__ heads) {
+  for (uint32_t o = blockIdx.x * 256u + threadIdx.x; o < n; o += gridDim.x * 256u) {
+    const uint32_t b = off[o];
+    if (off[o + 1] > b) heads[b] = o;
+  }
+}
+
+// Old slot j of source o: its neighbor's new id goes to position
+// j - off[o] of the new vertex's list (old order; sorted afterwards).
 __global__ __launch_bounds__(256) void k_rl_fill(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                                 uint32_t n, const int32_t *__restrict__ new_of_old,
+                                                 const uint32_t *__restrict__ src, uint64_t m,
+                                                 const int32_t *__restrict__ new_of_old,
                                                  const uint32_t *__restrict__ new_off, uint32_t *__restrict__ out) {
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * 256u + threadIdx.x) >> 6);
-  const uint32_t nwaves = gridDim.x * 4u;
-  const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t o = wave; o < n; o += nwaves) {
-    const uint32_t b = off[o], e = off[o + 1];
-    if (b == e) continue;
-    const uint32_t w = new_off[new_of_old[o]];
-    for (uint32_t i = b + lane; i < e; i += 64) out[w + (i - b)] = static_cast<uint32_t>(new_of_old[adj[i]]);
+  for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < m; j += gridDim.x * 256ull) {
+    const uint32_t o = src[j];
+    out[new_off[new_of_old[o]] + (j - off[o])] = static_cast<uint32_t>(new_of_old[adj[j]]);
   }
 }
 
@@ -69,7 +84,31 @@ int bit_width(uint32_t x) { return x ? 32 - __builtin_clz(x) : 1; }
 
 }  // namespace
 
-namespace accrelabel {
+namespace accprep {
+
+int slot_sources(acc_graph *g) {
+  const uint64_t m = g->m;
+  if (!m) return ACC_OK;
+  hipStream_t st = g->ctx->stream;
+  uint32_t *heads = g->d_rev;  // the reverse slots are built from the sources afterwards
+  uint32_t *src = reinterpret_cast<uint32_t *>(g->d_cnt);
+  size_t t_bytes = 0;
+  void *temp = nullptr;
+  hipError_t e = hipcub::DeviceScan::InclusiveScan(nullptr, t_bytes, heads, src, hipcub::Max(), m, st);
+  if (e == hipSuccess) e = hipMalloc(&temp, t_bytes ? t_bytes : 1);
+  if (e == hipSuccess) e = hipMemsetAsync(heads, 0, sizeof(uint32_t) * m, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rl_heads, dim3(rl_grid(g->n, g->ctx->num_cus)), dim3(256), 0, st, g->d_off, g->n, heads);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipcub::DeviceScan::InclusiveScan(temp, t_bytes, heads, src, hipcub::Max(), m, st);
+  (void)hipStreamSynchronize(st);
+  if (temp) (void)hipFree(temp);
+  if (e != hipSuccess)
+    return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
+                         std::string("slot sources: ") + hipGetErrorString(e));
+  return ACC_OK;
+}
 
 int relabel_degree(acc_graph *g) {
   const uint32_t n = g->n;
@@ -103,9 +142,12 @@ int relabel_degree(acc_graph *g) {
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&len), nb1);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&new_off), nb1);
   if (e != hipSuccess) return fail("allocation", e);
-  // cub temp storage: the largest of the three primitives' needs
+  // cub temp storage: the largest of the three primitives' needs.  d_cnt
+  // holds the slot sources (slot_sources); the reverse-slot array is free
+  // until the reverse slots are built and takes the mapped lists.
   size_t t_sort = 0, t_scan = 0, t_seg = 0;
-  uint32_t *mapped = reinterpret_cast<uint32_t *>(g->d_cnt);  // the count array is free until the first pass
+  uint32_t *mapped = g->d_rev;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(g->d_cnt);
   uint32_t *lists = reinterpret_cast<uint32_t *>(g->d_adj);
   e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t_sort, deg, deg_sorted, id, old_of_new,
                                                    static_cast<int>(n), 0, bits, st);
@@ -142,8 +184,8 @@ int relabel_degree(acc_graph *g) {
   if (e != hipSuccess) return fail("offsets", e);
   (void)record_event(g, "rl_offsets");
   if (m) {
-    hipLaunchKernelGGL(k_rl_fill, dim3(rl_grid(static_cast<uint64_t>(n) * 64, cus)), dim3(256), 0, st, g->d_off,
-                       g->d_adj, n, new_of_old, new_off, mapped);
+    hipLaunchKernelGGL(k_rl_fill, dim3(rl_grid(m, cus)), dim3(256), 0, st, g->d_off, g->d_adj, src, m, new_of_old,
+                       new_off, mapped);
     e = hipGetLastError();
     if (e == hipSuccess) (void)record_event(g, "rl_fill");
     if (e == hipSuccess)
@@ -163,4 +205,4 @@ int relabel_degree(acc_graph *g) {
   return ACC_OK;
 }
 
-}  // namespace accrelabel
+}  // namespace accprep

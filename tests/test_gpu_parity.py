@@ -170,19 +170,30 @@ def test_repeat_is_deterministic(ctx):
     assert np.array_equal(a, b) and ta == tb
 
 
-@pytest.mark.parametrize("case", ["selfloop", "unsorted", "asym", "range"])
-def test_malformed_upload_rejected(ctx, case):
+MALFORMED = {
+    "selfloop": ([0, 2, 4, 6], [0, 1, 0, 2, 1, 2]),
+    "unsorted": ([0, 2, 4, 6], [2, 1, 0, 2, 0, 1]),
+    "duplicate": ([0, 2, 4, 6], [1, 2, 0, 2, 0, 0]),
+    "range": ([0, 2, 4, 6], [1, 5, 0, 2, 0, 1]),
+    "asym_upper": ([0, 2, 3, 3], [1, 2, 0]),  # (0,2) has no reverse
+    "asym_lower": ([0, 1, 2, 3], [1, 0, 0]),  # (2,0) has no reverse; every upper slot has one
+}
+
+
+@pytest.mark.parametrize("case", sorted(MALFORMED))
+@pytest.mark.parametrize("relabel", [False, True])
+def test_malformed_upload_rejected(ctx, case, relabel):
     from acctricnt_amd import _ffi
 
-    off = np.array([0, 2, 4, 6], np.uint32)
-    adj = {"selfloop": [0, 1, 0, 2, 1, 2], "unsorted": [2, 1, 0, 2, 0, 1],
-           "asym": [1, 2, 0, 2, 1, 0], "range": [1, 5, 0, 2, 0, 1]}[case]
-    if case == "asym":
-        adj = [1, 2, 0, 2, 0, 0]
-    g = G.CSRGraph(off, np.array(adj, np.int32))
+    off, adj = MALFORMED[case]
+    g = G.CSRGraph(np.array(off, np.uint32), np.array(adj, np.int32))
     with pytest.raises(_ffi.AccError) as e:
-        ctx.upload(g)
+        ctx.upload(g, relabel=relabel)
     assert e.value.code == -2
+    msg = str(e.value)
+    want = {"selfloop": "Self loop", "unsorted": "not sorted", "duplicate": "not sorted",
+            "range": "out of range", "asym_upper": "not symmetric", "asym_lower": "not symmetric"}[case]
+    assert want in msg, msg
 
 
 def test_paths_agree_rmat20(ctx):
