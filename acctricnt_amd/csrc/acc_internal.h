@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,11 +31,20 @@ int record_event(acc_graph *g, const char *name);
     if (acc_rc_) return acc_rc_; \
   } while (0)
 
+// One lane of the staged host copies: a stream and two pinned buffers.
+struct CopyLane {
+  hipStream_t stream = nullptr;
+  void *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
 struct acc_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
   size_t lds_per_block = 0;
+  std::mutex copy_mu;             // the copy lanes serve one transfer at a time
+  std::vector<CopyLane> lanes;    // allocated at the first large pageable transfer
 };
 
 namespace accfwd {

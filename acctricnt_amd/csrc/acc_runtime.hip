@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <new>
 
 #include "acc_internal.h"
@@ -200,6 +201,121 @@ int record(acc_graph *g, bool timing, const char *name) {
   return ACC_OK;
 }
 
+// Copies between pageable host memory and the device near PCIe speed: the
+// transfer is split into kCopyLanes contiguous parts, each moved by a host
+// thread of its own through two pinned buffers of kStageBytes and a stream
+// of its own, so copying one chunk into (or out of) pinned memory overlaps
+// the DMA of the previous one.  Small transfers and host memory the device
+// can address already (pinned) go through one hipMemcpyAsync.
+constexpr int kCopyLanes = 4;
+constexpr size_t kStageBytes = size_t(16) << 20;
+constexpr size_t kStagedMin = size_t(8) << 20;
+
+bool host_pinned(const void *p) {
+  hipPointerAttribute_t attr;
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // pageable memory is not known to HIP: clear the error it leaves
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int ensure_copy_lanes(acc_ctx *ctx) {
+  if (!ctx->lanes.empty()) return ACC_OK;
+  std::vector<CopyLane> lanes(kCopyLanes);
+  hipError_t e = hipSuccess;
+  for (CopyLane &l : lanes) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+    for (int b = 0; b < 2 && e == hipSuccess; b++) {
+      e = hipHostMalloc(&l.buf[b], kStageBytes, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&l.ev[b], hipEventDisableTiming);
+    }
+  }
+  if (e != hipSuccess) {
+    for (CopyLane &l : lanes) {
+      for (int b = 0; b < 2; b++) {
+        if (l.buf[b]) (void)hipHostFree(l.buf[b]);
+        if (l.ev[b]) (void)hipEventDestroy(l.ev[b]);
+      }
+      if (l.stream) (void)hipStreamDestroy(l.stream);
+    }
+    return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
+                         std::string("copy lanes: ") + hipGetErrorString(e));
+  }
+  ctx->lanes = std::move(lanes);
+  return ACC_OK;
+}
+
+// to_device: host src -> device dst; else device src -> host dst.  Ordered
+// after the work already on ctx->stream; complete when it returns.
+int staged_copy(acc_ctx *ctx, void *dst, const void *src, size_t bytes, bool to_device) {
+  if (!bytes) return ACC_OK;
+  const void *host = to_device ? src : static_cast<const void *>(dst);
+  if (bytes < kStagedMin || host_pinned(host)) {
+    ACC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                               ctx->stream));
+    ACC_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ACC_OK;
+  }
+  std::lock_guard<std::mutex> lk(ctx->copy_mu);
+  ACC_TRY(ensure_copy_lanes(ctx));
+  hipEvent_t ready = nullptr;
+  ACC_HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+  hipError_t e = hipEventRecord(ready, ctx->stream);
+  const size_t part = ((bytes + kCopyLanes - 1) / kCopyLanes + 4095) / 4096 * 4096;
+  std::vector<hipError_t> errs(kCopyLanes, hipSuccess);
+  auto lane_main = [&](int li) {
+    CopyLane &L = ctx->lanes[li];
+    const size_t lo = std::min(bytes, part * li), hi = std::min(bytes, part * (li + 1));
+    hipError_t le = hipSetDevice(ctx->device);
+    if (le == hipSuccess) le = hipStreamWaitEvent(L.stream, ready, 0);
+    bool used[2] = {false, false};
+    size_t pend_off = 0, pend_len = 0;
+    int pend_b = -1;
+    int k = 0;
+    for (size_t o = lo; o < hi && le == hipSuccess; o += kStageBytes, k++) {
+      const size_t len = std::min(kStageBytes, hi - o);
+      const int b = k & 1;
+      if (to_device) {
+        if (used[b]) le = hipEventSynchronize(L.ev[b]);  // the chunk before last has left this buffer
+        if (le != hipSuccess) break;
+        std::memcpy(L.buf[b], static_cast<const char *>(src) + o, len);
+        le = hipMemcpyAsync(static_cast<char *>(dst) + o, L.buf[b], len, hipMemcpyHostToDevice, L.stream);
+        if (le == hipSuccess) le = hipEventRecord(L.ev[b], L.stream);
+        used[b] = true;
+      } else {
+        le = hipMemcpyAsync(L.buf[b], static_cast<const char *>(src) + o, len, hipMemcpyDeviceToHost, L.stream);
+        if (le == hipSuccess) le = hipEventRecord(L.ev[b], L.stream);
+        if (le == hipSuccess && pend_b >= 0) {  // the previous chunk is out of its buffer once its event fires
+          le = hipEventSynchronize(L.ev[pend_b]);
+          if (le == hipSuccess) std::memcpy(static_cast<char *>(dst) + pend_off, L.buf[pend_b], pend_len);
+        }
+        pend_b = b;
+        pend_off = o;
+        pend_len = len;
+      }
+    }
+    if (le == hipSuccess && !to_device && pend_b >= 0) {
+      le = hipEventSynchronize(L.ev[pend_b]);
+      if (le == hipSuccess) std::memcpy(static_cast<char *>(dst) + pend_off, L.buf[pend_b], pend_len);
+    }
+    const hipError_t se = hipStreamSynchronize(L.stream);
+    errs[li] = le != hipSuccess ? le : se;
+  };
+  if (e == hipSuccess) {
+    std::vector<std::thread> th;
+    for (int li = 1; li < kCopyLanes; li++) th.emplace_back(lane_main, li);
+    lane_main(0);
+    for (auto &t : th) t.join();
+    for (hipError_t le : errs)
+      if (le != hipSuccess) e = le;
+  }
+  (void)hipEventDestroy(ready);
+  if (e != hipSuccess) return acc_set_error(ACC_EDEVICE, std::string("staged copy: ") + hipGetErrorString(e));
+  return ACC_OK;
+}
+
 // Plan inputs of every vertex, computed on the device (k_vertex_meta): d_hs
 // stays resident, h_off, h_hs and h_lo are copied to the host for planning.
 // host_off, when given, is the caller's offsets array (equal to d_off).
@@ -300,6 +416,16 @@ void acc_ctx_destroy(acc_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
+  for (CopyLane &l : ctx->lanes) {
+    if (l.stream) {
+      (void)hipStreamSynchronize(l.stream);
+      (void)hipStreamDestroy(l.stream);
+    }
+    for (int b = 0; b < 2; b++) {
+      if (l.buf[b]) (void)hipHostFree(l.buf[b]);
+      if (l.ev[b]) (void)hipEventDestroy(l.ev[b]);
+    }
+  }
   delete ctx;
 }
 
@@ -339,11 +465,10 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
   if ((rc = dev_alloc(g, &g->d_rev, m + 4))) return bail(rc);
   if ((rc = dev_alloc(g, &g->d_sum, 2))) return bail(rc);
   if ((rc = dev_alloc(g, &g->d_flags, 4))) return bail(rc);
-  hipError_t e = hipMemcpyAsync(g->d_off, node_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
-                                hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess && m)
-    e = hipMemcpyAsync(g->d_adj, edge_dst, sizeof(int32_t) * m, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(g->d_flags, 0, sizeof(int32_t) * 4, ctx->stream);
+  if ((rc = staged_copy(ctx, g->d_off, node_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1), true)))
+    return bail(rc);
+  if ((rc = staged_copy(ctx, g->d_adj, edge_dst, sizeof(int32_t) * m, true))) return bail(rc);
+  hipError_t e = hipMemsetAsync(g->d_flags, 0, sizeof(int32_t) * 4, ctx->stream);
   if (e != hipSuccess) return bail(acc_set_error(ACC_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
   const bool relabel = (flags & ACC_UPLOAD_RELABEL) != 0;
   const bool validate = !(flags & ACC_UPLOAD_NO_VALIDATE) || relabel;  // the relabel indexes by neighbor id
@@ -390,8 +515,7 @@ int acc_graph_download(const acc_graph *g, uint32_t *node_off, int32_t *edge_dst
   DeviceGuard guard(g->ctx->device);
   hipStream_t st = g->ctx->stream;
   if (node_off) std::copy(g->h_off.begin(), g->h_off.end(), node_off);
-  if (edge_dst && g->m)
-    ACC_HIP_TRY(hipMemcpyAsync(edge_dst, g->d_adj, sizeof(int32_t) * g->m, hipMemcpyDeviceToHost, st));
+  if (edge_dst && g->m) ACC_TRY(staged_copy(g->ctx, edge_dst, g->d_adj, sizeof(int32_t) * g->m, false));
   if (old_id && g->d_old)
     ACC_HIP_TRY(hipMemcpyAsync(old_id, g->d_old, sizeof(int32_t) * g->n, hipMemcpyDeviceToHost, st));
   else if (old_id)
@@ -457,8 +581,7 @@ int acc_cn_download(acc_graph *g, int32_t *cnt_host, uint64_t *triangles) {
   if (!g) return acc_set_error(ACC_EINVAL, "null graph");
   if (!g->have_result) return acc_set_error(ACC_EINVAL, "no counts computed yet");
   DeviceGuard guard(g->ctx->device);
-  if (cnt_host && g->m)
-    ACC_HIP_TRY(hipMemcpyAsync(cnt_host, g->d_cnt, sizeof(int32_t) * g->m, hipMemcpyDeviceToHost, g->ctx->stream));
+  if (cnt_host && g->m) ACC_TRY(staged_copy(g->ctx, cnt_host, g->d_cnt, sizeof(int32_t) * g->m, false));
   unsigned long long s = 0;
   ACC_HIP_TRY(hipMemcpyAsync(&s, g->d_sum, sizeof(s), hipMemcpyDeviceToHost, g->ctx->stream));
   ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
