@@ -36,6 +36,7 @@ SIGNATURES = {
     "acc_graph_release": (None, [vp]),
     "acc_graph_info": (C.c_int, [vp, u32p, u64p, u64p]),
     "acc_graph_download": (C.c_int, [vp, u32p, i32p, i32p]),
+    "acc_graph_set_shard": (C.c_int, [vp, C.c_uint32, C.c_uint32, u32p]),
     "acc_cn_compute": (C.c_int, [vp, C.c_int, C.c_uint32]),
     "acc_cn_download": (C.c_int, [vp, i32p, u64p]),
     "acc_cn_all_edges": (C.c_int, [vp, C.c_int, C.c_uint32, i32p, u64p]),

@@ -184,6 +184,14 @@ class ResidentGraph:
         check_cn(self._lib.acc_graph_get_param(self._h, key, C.byref(v)))
         return int(v.value)
 
+    def set_shard(self, parts: int, index: int) -> tuple[int, int]:
+        """Restrict the forward path to share `index` of `parts` (equal probe
+        work, contiguous sources); returns the share's source range [lo, hi).
+        parts = 1 restores the whole graph."""
+        r = (C.c_uint32 * 2)()
+        check_cn(self._lib.acc_graph_set_shard(self._h, int(parts), int(index), r))
+        return int(r[0]), int(r[1])
+
     def set_partition(self, vertex_off) -> None:
         """Declare the graph a packed multi-graph CSR (graph k owns vertices
         [vertex_off[k], vertex_off[k+1])); counting passes then also reduce

@@ -118,6 +118,8 @@ struct acc_graph {
   uint32_t wave_low = accfwd::kWaveCap;      // ... and |N<(b)| at most this
   uint32_t chunk = accfwd::kChunk;           // upper slots per workgroup unit
   uint32_t persist = 0;                      // counting grid: 0 one workgroup per task, else per-CU workgroups
+  uint32_t shard_lo = 0, shard_hi = 0;       // forward sources [shard_lo, shard_hi) (the whole graph: 0, n)
+  std::vector<uint64_t> h_work;              // prefix of the forward probe work per source (n + 1), for shards
   // host copies kept for plan rebuilds: offsets, first upper slot and smallest
   // neighbor (the vertex itself when it has no lower neighbor) of every vertex
   std::vector<uint32_t> h_off;
@@ -251,6 +253,9 @@ int launch_graph_totals(acc_graph *g, int plus);
 int launch_validate(acc_graph *g);
 // Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v})).
 int launch_vertex_meta(acc_graph *g, int32_t *lo);
+// Forward work estimate of every source b into work[b]: the sum over its
+// upper slots (b, c) of |N(c) ∩ [0, b)| + 4, plus |N<(b)| + 256 per unit.
+int launch_source_work(acc_graph *g, unsigned long long *work);
 }  // namespace accfwd
 
 // Upload-time preparation (k_prepare.hip).  slot_sources writes the source

@@ -60,13 +60,13 @@ void free_graph(acc_graph *g) {
 // LDS bitmap when b - lo[b] <= kBitmapIds, else an LDS hash table); sources
 // whose N<(b) fits neither go to the global-memory variant.  hs[b] is the
 // first upper slot of b, lo[b] its smallest neighbor.
-void build_forward_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, const int32_t *lo, uint32_t wave_slots,
-                        uint32_t wave_low, uint32_t chunk, uint32_t wave_slots_on_device, std::vector<FwdUnit> &wave,
-                        std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
+void build_forward_plan(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const uint32_t *hs, const int32_t *lo,
+                        uint32_t wave_slots, uint32_t wave_low, uint32_t chunk, uint32_t wave_slots_on_device,
+                        std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
   // Sources per pack: just enough packs to give every wave slot of the
   // device one (a small graph keeps one source per wave), at most 64.
   uint64_t n_small = 0;
-  for (uint32_t b = 0; b < n; b++) {
+  for (uint32_t b = b_lo; b < b_hi; b++) {
     const uint32_t nl = hs[b] - off[b], nup = off[b + 1] - hs[b];
     n_small += nup && nup <= wave_slots && nl <= wave_low;
   }
@@ -85,7 +85,7 @@ void build_forward_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, con
     if (open) wave.push_back(pack);
     open = false;
   };
-  for (uint32_t b = 0; b < n; b++) {
+  for (uint32_t b = b_lo; b < b_hi; b++) {
     const uint32_t nl = hs[b] - off[b];
     const uint32_t nup = off[b + 1] - hs[b];
     if (nup == 0) continue;
@@ -156,7 +156,8 @@ int upload_plans(acc_graph *g) {
   drop_capture(g);
   std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
   const uint32_t device_waves = static_cast<uint32_t>(std::max(1, g->ctx->num_cus)) * 32u;
-  build_forward_plan(g->n, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots, g->wave_low, g->chunk,
+  build_forward_plan(g->shard_lo, g->shard_hi, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots,
+                     g->wave_low, g->chunk,
                      device_waves, wave, block, big);
   build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
   struct PlanList {
@@ -454,6 +455,7 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
   g->ctx = ctx;
   g->n = n;
   g->m = m;
+  g->shard_hi = n;
   int rc = ACC_OK;
   auto bail = [&](int code) {
     free_graph(g);
@@ -640,6 +642,7 @@ int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
       if (value < 64 || value > static_cast<int64_t>(accfwd::kChunk))
         return acc_set_error(ACC_EINVAL, "chunk must lie in [64, 512]");
       g->chunk = static_cast<uint32_t>(value);
+      g->h_work.clear();  // the shard work estimate counts units of `chunk` slots
       return upload_plans(g);
     case ACC_PARAM_PERSIST:
       if (value < 0 || value > 64) return acc_set_error(ACC_EINVAL, "persist must lie in [0, 64]");
@@ -759,6 +762,51 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap) {
     report[report_cap - 1] = '\0';
   }
   return ACC_OK;
+}
+
+int acc_graph_set_shard(acc_graph *g, uint32_t parts, uint32_t index, uint32_t *range_out) {
+  if (!g) return acc_set_error(ACC_EINVAL, "null graph");
+  if (parts == 0 || index >= parts) return acc_set_error(ACC_EINVAL, "need 0 <= index < parts");
+  DeviceGuard guard(g->ctx->device);
+  const uint32_t n = g->n;
+  uint32_t lo = 0, hi = n;
+  if (parts > 1 && n) {
+    if (g->h_work.empty()) {
+      unsigned long long *d_work = nullptr;
+      ACC_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_work), sizeof(unsigned long long) * n));
+      std::vector<unsigned long long> w(n);
+      int rc = accfwd::launch_source_work(g, d_work);
+      hipError_t e = hipSuccess;
+      if (!rc) e = hipMemcpyAsync(w.data(), d_work, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost,
+                                  g->ctx->stream);
+      if (!rc && e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+      (void)hipStreamSynchronize(g->ctx->stream);
+      (void)hipFree(d_work);
+      if (rc) return rc;
+      if (e != hipSuccess) return acc_set_error(ACC_EDEVICE, std::string("source work: ") + hipGetErrorString(e));
+      g->h_work.assign(static_cast<size_t>(n) + 1, 0);
+      for (uint32_t b = 0; b < n; b++) g->h_work[b + 1] = g->h_work[b] + w[b];
+    }
+    // share s = the sources whose work prefix lies in [s W / parts, (s + 1) W / parts)
+    const uint64_t total = g->h_work[n];
+    auto cut = [&](uint32_t s) -> uint32_t {
+      if (s == 0) return 0;
+      if (s == parts) return n;
+      const unsigned __int128 t = static_cast<unsigned __int128>(total) * s / parts;
+      const uint64_t target = static_cast<uint64_t>(t);
+      return static_cast<uint32_t>(std::lower_bound(g->h_work.begin(), g->h_work.end(), target) - g->h_work.begin());
+    };
+    lo = std::min(cut(index), n);
+    hi = std::max(lo, std::min(cut(index + 1), n));
+  }
+  if (range_out) {
+    range_out[0] = lo;
+    range_out[1] = hi;
+  }
+  if (lo == g->shard_lo && hi == g->shard_hi) return ACC_OK;
+  g->shard_lo = lo;
+  g->shard_hi = hi;
+  return upload_plans(g);
 }
 
 int acc_graph_set_partition(acc_graph *g, uint32_t num_graphs, const uint32_t *vertex_off) {

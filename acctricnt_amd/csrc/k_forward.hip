@@ -56,6 +56,28 @@ __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_
   if (tid == 0) *sum = 0;
 }
 
+// Forward probe work of every source b: over its upper slots (b, c) the
+// probe length |N(c) ∩ [0, b)| (the reverse slot minus the start of N(c))
+// plus 4 for the slot's fixed cost, and per unit of `chunk` upper slots the
+// build of the N<(b) table plus 256.  One wave per source.
+__global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict__ off, const uint32_t *__restrict__ hs,
+                                                     const int32_t *__restrict__ adj,
+                                                     const uint32_t *__restrict__ rev, uint32_t n, uint32_t chunk,
+                                                     unsigned long long *__restrict__ work) {
+  const uint32_t waves = gridDim.x * 4u;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6)); b < n; b += waves) {
+    unsigned long long w = 0;
+    for (uint32_t j = hs[b] + lane; j < off[b + 1]; j += 64) {
+      const uint32_t p = rev[j];
+      w += 4 + (p != ~0u ? p - off[adj[j]] : 0);
+    }
+    w = wave_sum_u64(w);
+    const uint32_t nup = off[b + 1] - hs[b], nl = hs[b] - off[b];
+    if (lane == 0) work[b] = w + static_cast<unsigned long long>((nup + chunk - 1) / chunk) * (nl + 256u);
+  }
+}
+
 // Plan inputs of every vertex v: hs[v], the first slot of N(v) holding an id
 // above v (the start of its upper slots), and lo[v] = min(N(v) ∪ {v}).
 __global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict__ off,
@@ -741,6 +763,16 @@ int launch_build_rev(acc_graph *g, bool validate) {
                      g->d_adj, reinterpret_cast<const uint32_t *>(g->d_cnt), g->n, g->m, g->d_rev,
                      validate ? g->d_flags : nullptr,
                      reinterpret_cast<unsigned long long *>(g->d_flags + 2));
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
+}
+
+int launch_source_work(acc_graph *g, unsigned long long *work) {
+  if (!g->n) return ACC_OK;
+  const uint64_t want = (static_cast<uint64_t>(g->n) + 3) / 4;
+  const uint64_t cap = static_cast<uint64_t>(g->ctx->num_cus > 0 ? g->ctx->num_cus : 256) * 16;
+  hipLaunchKernelGGL(k_source_work, dim3(static_cast<unsigned>(want < cap ? want : cap)), dim3(256), 0, g->ctx->stream,
+                     g->d_off, g->d_hs, g->d_adj, g->d_rev, g->n, g->chunk, work);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }

@@ -181,6 +181,45 @@ def bench_relabel(ctx, spec, relabelled, reps: int = 3):
     return res
 
 
+def bench_shards(ctx, g, parts: int = 8, reps: int = 3):
+    """The pass one GPU of a `parts`-GPU run would do: each share of equal
+    probe work (acc_graph_set_shard) timed alone with HIP events (best of
+    reps), with its algorithmic bytes (its sources' offsets and lists, their
+    upper slots' probes of the smaller list, and the two slot writes per
+    upper slot)."""
+    import numpy as np
+
+    src, dst = g.sources(), g.edge_dst
+    deg = g.degrees()
+    up = src < dst
+    probe = np.zeros(g.n, np.int64)
+    np.add.at(probe, src[up], np.minimum(deg[src[up]], deg[dst[up]]))
+    nup = np.bincount(src[up], minlength=g.n)
+    rg = ctx.upload(g)
+    shares = []
+    for s in range(parts):
+        lo, hi = rg.set_shard(parts, s)
+        rg.compute("forward")
+        best = None
+        for _ in range(reps):
+            rg.compute("forward", timing=True)
+            t = rg.timings()
+            if best is None or t["k_fwd_count<4>"] < best["k_fwd_count<4>"]:
+                best = t
+        b_alg = int(4 * (hi - lo + 1) + 4 * int(deg[lo:hi].sum()) + 4 * int(probe[lo:hi].sum())
+                    + 8 * int(nup[lo:hi].sum()))
+        k_ms = best["k_fwd_count<4>"]
+        shares.append({"sources": [lo, hi], "kernel_ms": k_ms, "pass_ms": sum(best.values()), "b_alg_bytes": b_alg,
+                       "frac_of_hbm_peak": b_alg / (k_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)})
+    rg.release()
+    worst = max(shares, key=lambda x: x["kernel_ms"])
+    res = {"parts": parts, "shares": shares, "max_kernel_ms": worst["kernel_ms"],
+           "mean_kernel_ms": sum(x["kernel_ms"] for x in shares) / parts}
+    print(f"[shards] {parts} shares: count kernel max {res['max_kernel_ms']:.3f} ms, "
+          f"mean {res['mean_kernel_ms']:.3f} ms", file=sys.stderr, flush=True)
+    return res
+
+
 def bench_batch(ctx, steps: int, warmup: int, num_graphs: int = 512):
     """Batched form on the 512-graph mix: one packed multi-graph CSR, autotuned,
     its counting pass captured in a HIP graph and replayed with one launch per
@@ -387,6 +426,8 @@ def main():
                     help="skip the whole-graph CPU bit-vector and triangle-count timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain measurement")
     ap.add_argument("--no-relabel", action="store_true", help="skip the device-relabel measurement")
+    ap.add_argument("--no-shards", action="store_true", help="skip the per-share (one GPU of several) measurement")
+    ap.add_argument("--shards", type=int, default=8, help="shares of the per-share measurement")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
     ap.add_argument("--serve-queries", type=int, default=6000)
@@ -480,6 +521,7 @@ def main():
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
     rg.release()
     relabel = None if args.no_relabel else bench_relabel(ctx, spec, g)
+    shards = None if args.no_shards or spec["kind"] == "batch" else bench_shards(ctx, g, args.shards)
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
     served = None if args.no_serve else bench_serve(
         ctx, args.serve_queries, args.serve_rate, streams=args.serve_streams,
@@ -528,6 +570,7 @@ def main():
                      "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
         "relabel": relabel,
+        "shards": shards,
         "batch": batch,
         "serve": served,
     }

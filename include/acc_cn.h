@@ -124,6 +124,22 @@ int acc_graph_get_param(const acc_graph *g, int param, int64_t *value);
 #define ACC_TUNE_FORWARD_ONLY 0x2u /* skip the bitmap and merge trials */
 int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap);
 
+/* Shards ------------------------------------------------------------------ */
+/* Restricts the forward path (ACC_PATH_FORWARD passes and the triangle-only
+ * pass) to share `index` of `parts`: the sources cut into `parts`
+ * contiguous vertex ranges of equal estimated work (an upper slot (b, c)
+ * weighs |N(c) ∩ [0, b)| + 4, a work unit of b its |N<(b)| + 256), the share
+ * one GPU of a `parts`-GPU run would count.
+ * A pass over a share zeroes every slot and counts the triangles whose middle
+ * vertex lies in the range; with ACC_CN_RAW the counts of all shares add up
+ * to the whole graph's, and their triangle totals to its total.  parts = 1
+ * restores the whole graph.  range_out (may be NULL) receives the share's
+ * [first, last + 1) source range.  Replaces the reference's per-GPU range
+ * split, which balances by sum over v in N(u), v > u of (d_v + 2)
+ * (ComputeVertexRangeAccumulatedFilteredDegree,
+ * scan-query-cuda/scan_xp_multi_gpu_multi_pass_common.cu:516-554). */
+int acc_graph_set_shard(acc_graph *g, uint32_t parts, uint32_t index, uint32_t *range_out);
+
 /* Multi-graph batches ----------------------------------------------------- */
 /* A packed multi-graph CSR is an ordinary CSR whose graphs occupy disjoint
  * vertex ranges: graph k owns vertices [vertex_off[k], vertex_off[k+1]) (int32
