@@ -181,6 +181,29 @@ def bench_relabel(ctx, spec, relabelled, reps: int = 3):
     return res
 
 
+def bench_scan_queries(rg, reps: int = 5):
+    """SCAN queries on the workload graph through a workspace (the drop-in's
+    path: similarity, cores, clusters, hubs, cluster ids, no index): device
+    time from enqueue to completion, best of reps, per (eps, mu)."""
+    out = {}
+    with rg.scan_workspace() as ws:
+        for eps, mu in (("0.2", 5), ("0.3", 5), ("0.6", 5)):
+            ws.run(eps, mu)
+            ws.synchronize()
+            best = None
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                ws.run(eps, mu)
+                ws.synchronize()
+                t = (time.perf_counter() - t0) * 1e3
+                best = t if best is None else min(best, t)
+            res = ws.results()
+            out[f"eps={eps},mu={mu}"] = {"ms": best, "cores": res["core_num"], "clusters": res["cluster_num"],
+                                         "hubs": res["hub_num"], "outliers": res["outlier_num"]}
+    print(f"[scan] {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_shards(ctx, g, parts: int = 8, reps: int = 3):
     """The pass one GPU of a `parts`-GPU run would do: each share of equal
     probe work (acc_graph_set_shard) timed alone with HIP events (best of
@@ -427,6 +450,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-graph batched-vs-plain measurement")
     ap.add_argument("--no-relabel", action="store_true", help="skip the device-relabel measurement")
     ap.add_argument("--no-shards", action="store_true", help="skip the per-share (one GPU of several) measurement")
+    ap.add_argument("--no-scan", action="store_true", help="skip the workspace SCAN query timings")
     ap.add_argument("--shards", type=int, default=8, help="shares of the per-share measurement")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
@@ -497,6 +521,7 @@ def main():
     tri_only_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     cnt, tri = rg.download()
+    scan = None if args.no_scan else bench_scan_queries(rg)
     rg.release()
     # end to end from host memory: upload (copy, validation, reverse slots,
     # plans), one pass, counts and total back in host memory
@@ -569,6 +594,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, args.workload),
                      "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
+        "scan_queries": scan,
         "relabel": relabel,
         "shards": shards,
         "batch": batch,
