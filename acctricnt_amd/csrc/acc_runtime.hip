@@ -306,8 +306,13 @@ int staged_copy(acc_ctx *ctx, void *dst, const void *src, size_t bytes, bool to_
   };
   if (e == hipSuccess) {
     std::vector<std::thread> th;
-    for (int li = 1; li < kCopyLanes; li++) th.emplace_back(lane_main, li);
+    int spawned = 1;
+    try {
+      for (; spawned < kCopyLanes; spawned++) th.emplace_back(lane_main, spawned);
+    } catch (...) {  // no thread to spare: the remaining lanes run on this one
+    }
     lane_main(0);
+    for (int li = spawned; li < kCopyLanes; li++) lane_main(li);
     for (auto &t : th) t.join();
     for (hipError_t le : errs)
       if (le != hipSuccess) e = le;
