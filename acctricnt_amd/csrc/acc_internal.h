@@ -254,7 +254,8 @@ int launch_validate(acc_graph *g);
 // Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v})).
 int launch_vertex_meta(acc_graph *g, int32_t *lo);
 // Forward work estimate of every source b into work[b]: the sum over its
-// upper slots (b, c) of |N(c) ∩ [0, b)| + 4, plus |N<(b)| + 256 per unit.
+// upper slots (b, c) of |N(c) ∩ [0, b)| + 4 + 2 hits, plus |N<(b)| + 256 per
+// unit, hits read from d_cnt's upper slots after a pass without the combine.
 int launch_source_work(acc_graph *g, unsigned long long *work);
 }  // namespace accfwd
 

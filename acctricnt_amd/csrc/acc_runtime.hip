@@ -777,6 +777,18 @@ int acc_graph_set_shard(acc_graph *g, uint32_t parts, uint32_t index, uint32_t *
   uint32_t lo = 0, hi = n;
   if (parts > 1 && n) {
     if (g->h_work.empty()) {
+      // one whole-graph counting pass without the combine leaves in every
+      // upper slot (b, c) its probe's hits, which the work estimate weighs
+      if (g->shard_lo != 0 || g->shard_hi != n) {
+        g->shard_lo = 0;
+        g->shard_hi = n;
+        ACC_TRY(upload_plans(g));
+      }
+      g->have_result = false;
+      g->totals_valid = false;
+      ACC_TRY(accfwd::launch_init(g));
+      ACC_TRY(accfwd::launch_count(g));
+      if (g->n_fwd_big) ACC_TRY(accfwd::launch_count_big(g));
       unsigned long long *d_work = nullptr;
       ACC_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&d_work), sizeof(unsigned long long) * n));
       std::vector<unsigned long long> w(n);

@@ -56,13 +56,16 @@ __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_
   if (tid == 0) *sum = 0;
 }
 
-// Forward probe work of every source b: over its upper slots (b, c) the
+// Forward work estimate of every source b: over its upper slots (b, c) the
 // probe length |N(c) ∩ [0, b)| (the reverse slot minus the start of N(c))
-// plus 4 for the slot's fixed cost, and per unit of `chunk` upper slots the
-// build of the N<(b) table plus 256.  One wave per source.
+// plus 4 for the slot's fixed cost plus twice the probe's hits (hits[j],
+// the triangles with middle b and top c: the upper slot's value after a
+// counting pass without the combine), and per unit of `chunk` upper slots
+// the build of the N<(b) table plus 256.  One wave per source.
 __global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict__ off, const uint32_t *__restrict__ hs,
                                                      const int32_t *__restrict__ adj,
-                                                     const uint32_t *__restrict__ rev, uint32_t n, uint32_t chunk,
+                                                     const uint32_t *__restrict__ rev, const int32_t *__restrict__ hits,
+                                                     uint32_t n, uint32_t chunk,
                                                      unsigned long long *__restrict__ work) {
   const uint32_t waves = gridDim.x * 4u;
   const int lane = threadIdx.x & 63;
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict_
     unsigned long long w = 0;
     for (uint32_t j = hs[b] + lane; j < off[b + 1]; j += 64) {
       const uint32_t p = rev[j];
-      w += 4 + (p != ~0u ? p - off[adj[j]] : 0);
+      w += 4 + (p != ~0u ? p - off[adj[j]] : 0) + 2ull * static_cast<uint32_t>(hits[j]);
     }
     w = wave_sum_u64(w);
     const uint32_t nup = off[b + 1] - hs[b], nl = hs[b] - off[b];
@@ -772,7 +775,7 @@ int launch_source_work(acc_graph *g, unsigned long long *work) {
   const uint64_t want = (static_cast<uint64_t>(g->n) + 3) / 4;
   const uint64_t cap = static_cast<uint64_t>(g->ctx->num_cus > 0 ? g->ctx->num_cus : 256) * 16;
   hipLaunchKernelGGL(k_source_work, dim3(static_cast<unsigned>(want < cap ? want : cap)), dim3(256), 0, g->ctx->stream,
-                     g->d_off, g->d_hs, g->d_adj, g->d_rev, g->n, g->chunk, work);
+                     g->d_off, g->d_hs, g->d_adj, g->d_rev, g->d_cnt, g->n, g->chunk, work);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }

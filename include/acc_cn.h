@@ -128,8 +128,10 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap);
 /* Restricts the forward path (ACC_PATH_FORWARD passes and the triangle-only
  * pass) to share `index` of `parts`: the sources cut into `parts`
  * contiguous vertex ranges of equal estimated work (an upper slot (b, c)
- * weighs |N(c) ∩ [0, b)| + 4, a work unit of b its |N<(b)| + 256), the share
- * one GPU of a `parts`-GPU run would count.
+ * weighs |N(c) ∩ [0, b)| + 4 + twice the triangles with middle b and top c,
+ * a work unit of b its |N<(b)| + 256), the share one GPU of a `parts`-GPU
+ * run would count.  The first call with parts > 1 runs one counting pass to
+ * measure the triangles (the counts are then undefined until the next pass).
  * A pass over a share zeroes every slot and counts the triangles whose middle
  * vertex lies in the range; with ACC_CN_RAW the counts of all shares add up
  * to the whole graph's, and their triangle totals to its total.  parts = 1
