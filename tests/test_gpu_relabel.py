@@ -101,3 +101,15 @@ def test_relabel_full_size_rmat22(ctx):
         t_relabel = rg.triangle_total()
     with ctx.upload(g) as rg:
         assert rg.triangle_total() == t_relabel
+
+
+def test_relabel_full_size_chunglu23(ctx):
+    """Chung-Lu 2^23 as generated: the device relabel equals the host relabel."""
+    g = G.gen_chung_lu(1 << 23, 2.1, 32, 1 << 20, seed=1, relabel=False)
+    print(f"[relabel chunglu23] n={g.n} m={g.m}", flush=True)
+    h = G.relabel_degree(g)
+    with ctx.upload(g, relabel=True) as rg:
+        r = rg.csr()
+    assert np.array_equal(r.node_off, h.node_off)
+    assert np.array_equal(r.edge_dst, h.edge_dst)
+    assert np.array_equal(r.old_of_new, h.old_of_new)
