@@ -114,7 +114,8 @@ int relabel_degree(acc_graph *g) {
   const uint32_t n = g->n;
   const uint64_t m = g->m;
   if (n == 0) return ACC_OK;
-  if (m >= (1ULL << 31)) return acc_set_error(ACC_EINVAL, "the device relabel needs m < 2^31");
+  if (m >= (1ULL << 31) || n >= (1u << 31) - 1)
+    return acc_set_error(ACC_EINVAL, "the device relabel needs m < 2^31 and n < 2^31 - 1");
   hipStream_t st = g->ctx->stream;
   const int cus = g->ctx->num_cus;
   const int bits = bit_width(n - 1);  // ids and degrees are below n
