@@ -183,6 +183,29 @@ def bench_relabel(ctx, spec, relabelled, reps: int = 3):
     return res
 
 
+def bench_other_paths(rg, b_alg: int, ref_counts):
+    """One timed pass (after a warm-up pass) of each per-edge path, the
+    reference's two GPU algorithms rebuilt (ACC_PATH_BITMAP, ACC_PATH_MERGE),
+    at their default unit size (512 upper slots), with counts checked against
+    the forward path's."""
+    import numpy as np
+
+    out = {}
+    chunk = rg.get_param("chunk")
+    rg.set_param("chunk", 512)
+    for path in ("bitmap", "merge"):
+        rg.compute(path)
+        rg.compute(path, timing=True)
+        t = rg.timings()
+        cnt, _ = rg.download()
+        ms = sum(t.values())
+        out[path] = {"ms": ms, "kernel_ms": t, "frac_of_hbm_peak": b_alg / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                     "counts_equal_forward": bool(np.array_equal(cnt, ref_counts))}
+    rg.set_param("chunk", chunk)
+    print(f"[paths] bitmap {out['bitmap']['ms']:.1f} ms, merge {out['merge']['ms']:.1f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_scan_queries(rg, reps: int = 5):
     """SCAN queries on the workload graph through a workspace (the drop-in's
     path: similarity, cores, clusters, hubs, cluster ids, no index): device
@@ -455,6 +478,7 @@ def main():
     ap.add_argument("--no-relabel", action="store_true", help="skip the device-relabel measurement")
     ap.add_argument("--no-shards", action="store_true", help="skip the per-share (one GPU of several) measurement")
     ap.add_argument("--no-scan", action="store_true", help="skip the workspace SCAN query timings")
+    ap.add_argument("--no-paths", action="store_true", help="skip the per-edge bitmap and merge path timings")
     ap.add_argument("--shards", type=int, default=8, help="shares of the per-share measurement")
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--no-serve", action="store_true", help="skip the SCAN serving measurement")
@@ -527,6 +551,7 @@ def main():
 
     cnt, tri = rg.download()
     scan = None if args.no_scan else bench_scan_queries(rg)
+    other_paths = None if args.no_paths or spec["kind"] == "batch" else bench_other_paths(rg, b_alg, cnt)
     rg.release()
     # end to end from host memory: upload (copy, validation, reverse slots,
     # plans), one pass, counts and total back in host memory
@@ -600,6 +625,7 @@ def main():
                      "kernel": dom, "kernel_ms": per_kernel[dom], "algorithmic_bytes": b_alg},
         "cpu_baseline": cpu,
         "scan_queries": scan,
+        "other_paths": other_paths,
         "relabel": relabel,
         "shards": shards,
         "batch": batch,
