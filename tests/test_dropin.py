@@ -106,3 +106,25 @@ def test_triangle_count_vs_oracle(tmp_path):
         assert r.returncode == 0, r.stderr
     counts = [int(re.sub(r"[^0-9]", "", m)) for m in re.findall(r"Triangle Cnt: ([0-9][0-9,.]*)", log.read_text())]
     assert counts == [oracle.triangle_count(g)] * 2
+
+
+@pytest.mark.gpu
+def test_scan_result_file_full_size_rmat22(tmp_path):
+    """R-MAT scale 22 as generated, through the drop-in with the deg reorder
+    (relabelled on the GPU): the result file equals the oracle's SCAN of the
+    graph in its original ids, byte for byte (counts from the oracle's
+    bit-vector path, independent of the GPU)."""
+    g = G.gen_rmat(22, 16, seed=1, relabel=False)
+    print(f"[drop-in rmat22] n={g.n} m={g.m}", flush=True)
+    G.write_bins(g, str(tmp_path))
+    eps, mu = "0.2", 5
+    log = tmp_path / "log.txt"
+    r = run("main_scanxp", tmp_path, eps, mu, 16, "deg", log, timeout=280)
+    assert r.returncode == 0, r.stderr
+    ref = oracle.scan(g, oracle.cn_bitmap(g), eps, mu)
+    expect = tmp_path / "expect.txt"
+    oracle.write_result(str(expect), g, ref)
+    out = tmp_path / f"scanxp-result-{oracle.eps_string(float(eps))}-{mu}.txt"
+    assert out.read_bytes() == expect.read_bytes()
+    text = log.read_text()
+    assert _count(text, "Core") == ref["core_num"]
