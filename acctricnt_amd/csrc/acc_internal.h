@@ -107,6 +107,7 @@ struct acc_graph {
   uint32_t n_bmp_lds = 0;
   FwdUnit *d_bmp_hub = nullptr;  // hub sources (bitmap path, HBM bitmap)
   uint32_t n_bmp_hub = 0;
+  bool edge_plans_valid = false;  // per-edge plans are built at the first bitmap or merge pass
   uint32_t *d_bmp_pool = nullptr;  // one zeroed n-bit bitmap per persistent workgroup
   uint32_t bmp_pool_slots = 0;
   uint64_t bmp_pool_words = 0;

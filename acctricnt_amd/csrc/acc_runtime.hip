@@ -150,24 +150,16 @@ void drop_capture(acc_graph *g) {
   g->graph = nullptr;
 }
 
-// (Re)builds every work plan from the host split points and the current bin
-// parameters, and uploads the unit lists.
-int upload_plans(acc_graph *g) {
-  drop_capture(g);
-  std::vector<FwdUnit> wave, block, big, eall, elds, ehub;
-  const uint32_t device_waves = static_cast<uint32_t>(std::max(1, g->ctx->num_cus)) * 32u;
-  build_forward_plan(g->shard_lo, g->shard_hi, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots,
-                     g->wave_low, g->chunk,
-                     device_waves, wave, block, big);
-  build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
-  struct PlanList {
-    std::vector<FwdUnit> *v;
-    FwdUnit **d;
-    uint32_t *count;
-  };
-  PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},    {&block, &g->d_fwd_block, &g->n_fwd_block},
-                      {&big, &g->d_fwd_big, &g->n_fwd_big},       {&eall, &g->d_edge_units, &g->n_edge_units},
-                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
+// A host unit list and the device copy it replaces.
+struct PlanList {
+  std::vector<FwdUnit> *v;
+  FwdUnit **d;
+  uint32_t *count;
+};
+
+// Replaces the device copies of the given unit lists.
+template <size_t N>
+int upload_lists(acc_graph *g, PlanList (&lists)[N]) {
   ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
   for (auto &l : lists) {
     dev_free(g, l.d, *l.count);
@@ -178,6 +170,35 @@ int upload_plans(acc_graph *g) {
                                g->ctx->stream));
   }
   ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  return ACC_OK;
+}
+
+// Rebuilds the forward plan from the host split points, the current bin
+// parameters and the shard; the per-edge plans are rebuilt at their next use.
+int upload_plans(acc_graph *g) {
+  drop_capture(g);
+  std::vector<FwdUnit> wave, block, big;
+  const uint32_t device_waves = static_cast<uint32_t>(std::max(1, g->ctx->num_cus)) * 32u;
+  build_forward_plan(g->shard_lo, g->shard_hi, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots,
+                     g->wave_low, g->chunk, device_waves, wave, block, big);
+  PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},
+                      {&block, &g->d_fwd_block, &g->n_fwd_block},
+                      {&big, &g->d_fwd_big, &g->n_fwd_big}};
+  g->edge_plans_valid = false;
+  return upload_lists(g, lists);
+}
+
+// The per-edge (bitmap and merge) plans, built at the first pass that needs
+// them: most graphs only ever run the forward path.
+int ensure_edge_plans(acc_graph *g) {
+  if (g->edge_plans_valid) return ACC_OK;
+  std::vector<FwdUnit> eall, elds, ehub;
+  build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
+  PlanList lists[] = {{&eall, &g->d_edge_units, &g->n_edge_units},
+                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},
+                      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
+  ACC_TRY(upload_lists(g, lists));
+  g->edge_plans_valid = true;
   return ACC_OK;
 }
 
@@ -329,8 +350,8 @@ int load_vertex_meta(acc_graph *g, const uint32_t *host_off) {
   const uint32_t n = g->n;
   ACC_TRY(dev_alloc(g, &g->d_hs, static_cast<uint64_t>(n) + 1));
   int32_t *lo = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void **>(&lo), sizeof(int32_t) * (n ? n : 1));
-  if (e != hipSuccess) return acc_set_error(ACC_ENOMEM, "hipMalloc of the vertex metadata failed");
+  if (hipMalloc(reinterpret_cast<void **>(&lo), sizeof(int32_t) * (n ? n : 1)) != hipSuccess)
+    return acc_set_error(ACC_ENOMEM, "hipMalloc of the vertex metadata failed");
   g->h_off.resize(static_cast<size_t>(n) + 1);
   g->h_hs.resize(n);
   g->h_lo.resize(n);
@@ -340,14 +361,10 @@ int load_vertex_meta(acc_graph *g, const uint32_t *host_off) {
     if (host_off)
       std::copy(host_off, host_off + static_cast<size_t>(n) + 1, g->h_off.begin());
     else
-      e = hipMemcpyAsync(g->h_off.data(), g->d_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1),
-                         hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && n)
-      e = hipMemcpyAsync(g->h_hs.data(), g->d_hs, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && n) e = hipMemcpyAsync(g->h_lo.data(), lo, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) rc = acc_set_error(ACC_EDEVICE, std::string("vertex metadata: ") + hipGetErrorString(e));
+      rc = staged_copy(g->ctx, g->h_off.data(), g->d_off, sizeof(uint32_t) * (static_cast<size_t>(n) + 1), false);
   }
+  if (!rc) rc = staged_copy(g->ctx, g->h_hs.data(), g->d_hs, sizeof(uint32_t) * n, false);
+  if (!rc) rc = staged_copy(g->ctx, g->h_lo.data(), lo, sizeof(int32_t) * n, false);
   (void)hipStreamSynchronize(st);
   (void)hipFree(lo);
   if (rc) return rc;
@@ -543,6 +560,7 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   g->events.clear();
   g->event_names.clear();
   g->timing_valid = false;
+  if (path == ACC_PATH_BITMAP || path == ACC_PATH_MERGE) ACC_TRY(ensure_edge_plans(g));
   if (path == ACC_PATH_BITMAP) ACC_TRY(ensure_bitmap_pool(g));
   ACC_TRY(record(g, timing, "begin"));
   if (path == ACC_PATH_BITMAP || path == ACC_PATH_MERGE) {
@@ -881,7 +899,9 @@ int acc_cn_capture(acc_graph *g, int path, uint32_t flags) {
   if (flags & ACC_CN_TIMING) return acc_set_error(ACC_EINVAL, "timing events cannot be captured");
   DeviceGuard guard(g->ctx->device);
   if (path == ACC_PATH_AUTO) path = g->auto_path;
-  if (path == ACC_PATH_BITMAP) ACC_TRY(ensure_bitmap_pool(g));  // allocates and clears outside the capture
+  // plans and the bitmap pool are allocated and cleared outside the capture
+  if (path == ACC_PATH_BITMAP || path == ACC_PATH_MERGE) ACC_TRY(ensure_edge_plans(g));
+  if (path == ACC_PATH_BITMAP) ACC_TRY(ensure_bitmap_pool(g));
   ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
   drop_capture(g);
   ACC_HIP_TRY(hipStreamBeginCapture(g->ctx->stream, hipStreamCaptureModeThreadLocal));
