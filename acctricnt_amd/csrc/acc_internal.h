@@ -252,8 +252,9 @@ int launch_combine(acc_graph *g, int plus);
 int launch_graph_totals(acc_graph *g, int plus);
 // Slot-parallel over the sources in d_cnt (accprep::slot_sources).
 int launch_validate(acc_graph *g);
-// Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v})).
-int launch_vertex_meta(acc_graph *g, int32_t *lo);
+// Per-vertex plan inputs into g->d_hs (first upper slot) and lo (min(N(v) ∪ {v}));
+// maxes[0] = max |N<(v)|, maxes[1] = max degree.
+int launch_vertex_meta(acc_graph *g, int32_t *lo, uint32_t *maxes);
 // Forward work estimate of every source b into work[b]: the sum over its
 // upper slots (b, c) of |N(c) ∩ [0, b)| + 4 + 2 hits, plus |N<(b)| + 256 per
 // unit, hits read from d_cnt's upper slots after a pass without the combine.

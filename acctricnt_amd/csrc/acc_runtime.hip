@@ -60,18 +60,9 @@ void free_graph(acc_graph *g) {
 // LDS bitmap when b - lo[b] <= kBitmapIds, else an LDS hash table); sources
 // whose N<(b) fits neither go to the global-memory variant.  hs[b] is the
 // first upper slot of b, lo[b] its smallest neighbor.
-void build_forward_plan(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const uint32_t *hs, const int32_t *lo,
-                        uint32_t wave_slots, uint32_t wave_low, uint32_t chunk, uint32_t wave_slots_on_device,
-                        std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
-  // Sources per pack: just enough packs to give every wave slot of the
-  // device one (a small graph keeps one source per wave), at most 64.
-  uint64_t n_small = 0;
-  for (uint32_t b = b_lo; b < b_hi; b++) {
-    const uint32_t nl = hs[b] - off[b], nup = off[b + 1] - hs[b];
-    n_small += nup && nup <= wave_slots && nl <= wave_low;
-  }
-  const uint64_t per = (n_small + wave_slots_on_device - 1) / std::max<uint32_t>(wave_slots_on_device, 1);
-  const uint32_t max_sources = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, per)));
+void build_forward_range(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const uint32_t *hs, const int32_t *lo,
+                         uint32_t wave_slots, uint32_t wave_low, uint32_t chunk, uint32_t max_sources,
+                         std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
   auto table = [](uint32_t nl) -> uint32_t {
     if (!nl) return 0;
     uint32_t t = 16;
@@ -116,6 +107,58 @@ void build_forward_plan(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const
     }
   }
   close();
+}
+
+// Builds the plan of sources [b_lo, b_hi) in kPlanThreads contiguous pieces
+// on as many host threads (packs do not cross a piece boundary) and joins the
+// pieces' lists in order.  Sources per pack: just enough packs to give every
+// wave slot of the device one (a small graph keeps one source per wave), at
+// most 64.
+constexpr uint32_t kPlanThreads = 8;
+void build_forward_plan(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const uint32_t *hs, const int32_t *lo,
+                        uint32_t wave_slots, uint32_t wave_low, uint32_t chunk, uint32_t wave_slots_on_device,
+                        std::vector<FwdUnit> &wave, std::vector<FwdUnit> &block, std::vector<FwdUnit> &big) {
+  const uint32_t span = b_hi - b_lo;
+  const uint32_t pieces = span < (1u << 16) ? 1 : kPlanThreads;
+  std::vector<uint64_t> small(pieces, 0);
+  std::vector<std::vector<FwdUnit>> w(pieces), bl(pieces), bg(pieces);
+  auto piece = [&](uint32_t t) {
+    return std::pair<uint32_t, uint32_t>(b_lo + static_cast<uint32_t>(uint64_t(span) * t / pieces),
+                                         b_lo + static_cast<uint32_t>(uint64_t(span) * (t + 1) / pieces));
+  };
+  auto run = [&](auto &&fn) {
+    std::vector<std::thread> th;
+    uint32_t t = 1;
+    try {
+      for (; t < pieces; t++) th.emplace_back(fn, t);
+    } catch (...) {  // no thread to spare: the rest run here
+    }
+    fn(0);
+    for (uint32_t r = t; r < pieces; r++) fn(r);
+    for (auto &x : th) x.join();
+  };
+  run([&](uint32_t t) {
+    const auto [a, e] = piece(t);
+    uint64_t c = 0;
+    for (uint32_t b = a; b < e; b++) {
+      const uint32_t nl = hs[b] - off[b], nup = off[b + 1] - hs[b];
+      c += nup && nup <= wave_slots && nl <= wave_low;
+    }
+    small[t] = c;
+  });
+  uint64_t n_small = 0;
+  for (uint64_t c : small) n_small += c;
+  const uint64_t per = (n_small + wave_slots_on_device - 1) / std::max<uint32_t>(wave_slots_on_device, 1);
+  const uint32_t max_sources = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, per)));
+  run([&](uint32_t t) {
+    const auto [a, e] = piece(t);
+    build_forward_range(a, e, off, hs, lo, wave_slots, wave_low, chunk, max_sources, w[t], bl[t], bg[t]);
+  });
+  for (uint32_t t = 0; t < pieces; t++) {
+    wave.insert(wave.end(), w[t].begin(), w[t].end());
+    block.insert(block.end(), bl[t].begin(), bl[t].end());
+    big.insert(big.end(), bg[t].begin(), bg[t].end());
+  }
 }
 
 // Per-edge plan: chunks of at most `chunk` upper slots of every source (merge
@@ -350,12 +393,14 @@ int load_vertex_meta(acc_graph *g, const uint32_t *host_off) {
   const uint32_t n = g->n;
   ACC_TRY(dev_alloc(g, &g->d_hs, static_cast<uint64_t>(n) + 1));
   int32_t *lo = nullptr;
-  if (hipMalloc(reinterpret_cast<void **>(&lo), sizeof(int32_t) * (n ? n : 1)) != hipSuccess)
+  if (hipMalloc(reinterpret_cast<void **>(&lo), sizeof(int32_t) * (static_cast<size_t>(n) + 2)) != hipSuccess)
     return acc_set_error(ACC_ENOMEM, "hipMalloc of the vertex metadata failed");
+  uint32_t *d_max = reinterpret_cast<uint32_t *>(lo + n);
+  uint32_t h_max[2] = {0, 0};
   g->h_off.resize(static_cast<size_t>(n) + 1);
   g->h_hs.resize(n);
   g->h_lo.resize(n);
-  int rc = accfwd::launch_vertex_meta(g, lo);
+  int rc = accfwd::launch_vertex_meta(g, lo, d_max);
   hipStream_t st = g->ctx->stream;
   if (!rc) {
     if (host_off)
@@ -365,15 +410,12 @@ int load_vertex_meta(acc_graph *g, const uint32_t *host_off) {
   }
   if (!rc) rc = staged_copy(g->ctx, g->h_hs.data(), g->d_hs, sizeof(uint32_t) * n, false);
   if (!rc) rc = staged_copy(g->ctx, g->h_lo.data(), lo, sizeof(int32_t) * n, false);
+  if (!rc) rc = staged_copy(g->ctx, h_max, d_max, sizeof(h_max), false);
   (void)hipStreamSynchronize(st);
   (void)hipFree(lo);
   if (rc) return rc;
-  g->max_low = 0;
-  g->max_deg = 0;
-  for (uint32_t v = 0; v < n; v++) {
-    g->max_low = std::max(g->max_low, g->h_hs[v] - g->h_off[v]);
-    g->max_deg = std::max(g->max_deg, g->h_off[v + 1] - g->h_off[v]);
-  }
+  g->max_low = h_max[0];
+  g->max_deg = h_max[1];
   return ACC_OK;
 }
 

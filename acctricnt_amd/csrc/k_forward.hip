@@ -83,9 +83,12 @@ __global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict_
 
 // Plan inputs of every vertex v: hs[v], the first slot of N(v) holding an id
 // above v (the start of its upper slots), and lo[v] = min(N(v) ∪ {v}).
+// maxes (zeroed first) gets max |N<(v)| and the maximum degree.
 __global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict__ off,
                                                      const int32_t *__restrict__ adj, uint32_t n,
-                                                     uint32_t *__restrict__ hs, int32_t *__restrict__ lo) {
+                                                     uint32_t *__restrict__ hs, int32_t *__restrict__ lo,
+                                                     uint32_t *__restrict__ maxes) {
+  uint32_t max_low = 0, max_deg = 0;
   for (uint32_t v = blockIdx.x * 256u + threadIdx.x; v < n; v += gridDim.x * 256u) {
     const uint32_t s = off[v], e = off[v + 1];
     uint32_t a = s, b = e;
@@ -95,6 +98,17 @@ __global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict_
     }
     hs[v] = a;
     lo[v] = e > s ? min(adj[s], static_cast<int32_t>(v)) : static_cast<int32_t>(v);
+    max_low = max(max_low, a - s);
+    max_deg = max(max_deg, e - s);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    max_low = max(max_low, static_cast<uint32_t>(__shfl_xor(static_cast<int>(max_low), o, 64)));
+    max_deg = max(max_deg, static_cast<uint32_t>(__shfl_xor(static_cast<int>(max_deg), o, 64)));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (max_low) atomicMax(&maxes[0], max_low);
+    if (max_deg) atomicMax(&maxes[1], max_deg);
   }
 }
 
@@ -780,10 +794,11 @@ int launch_source_work(acc_graph *g, unsigned long long *work) {
   return ACC_OK;
 }
 
-int launch_vertex_meta(acc_graph *g, int32_t *lo) {
+int launch_vertex_meta(acc_graph *g, int32_t *lo, uint32_t *maxes) {
+  ACC_HIP_TRY(hipMemsetAsync(maxes, 0, 2 * sizeof(uint32_t), g->ctx->stream));
   if (!g->n) return ACC_OK;
   hipLaunchKernelGGL(k_vertex_meta, dim3(grid_for(g->n, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_off,
-                     g->d_adj, g->n, g->d_hs, lo);
+                     g->d_adj, g->n, g->d_hs, lo, maxes);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
