@@ -152,6 +152,16 @@ def test_serve_errors(ctx, graphs):
         sv.release(t)
         with pytest.raises(_ffi.AccError):
             sv.result(t)
+        # served graphs take no upload flag but ACC_UPLOAD_NO_VALIDATE (relabel before adding)
+        import ctypes as C
+
+        g4 = gs[4]
+        off = np.ascontiguousarray(g4.node_off, np.uint32)
+        adj = np.ascontiguousarray(g4.edge_dst, np.int32)
+        gid_out = C.c_int32()
+        rc = sv._lib.acc_server_add_graph(sv._h, g4.n, g4.m, _ffi.ptr(off, C.c_uint32), _ffi.ptr(adj, C.c_int32),
+                                          cn.UPLOAD_RELABEL, C.byref(gid_out))
+        assert rc == -1
         # a malformed graph is rejected at registration
         bad = G.CSRGraph(np.array([0, 2, 4, 6], np.uint32), np.array([2, 1, 0, 2, 0, 1], np.int32))
         with pytest.raises(_ffi.AccError) as e:
