@@ -317,14 +317,15 @@ int ensure_copy_lanes(acc_ctx *ctx) {
 int staged_copy(acc_ctx *ctx, void *dst, const void *src, size_t bytes, bool to_device) {
   if (!bytes) return ACC_OK;
   const void *host = to_device ? src : static_cast<const void *>(dst);
-  if (bytes < kStagedMin || host_pinned(host)) {
+  auto direct = [&]() {
     ACC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
                                ctx->stream));
     ACC_HIP_TRY(hipStreamSynchronize(ctx->stream));
     return ACC_OK;
-  }
+  };
+  if (bytes < kStagedMin || host_pinned(host)) return direct();
   std::lock_guard<std::mutex> lk(ctx->copy_mu);
-  ACC_TRY(ensure_copy_lanes(ctx));
+  if (ensure_copy_lanes(ctx) != ACC_OK) return direct();  // no pinned memory to spare: copy directly
   hipEvent_t ready = nullptr;
   ACC_HIP_TRY(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
   hipError_t e = hipEventRecord(ready, ctx->stream);
