@@ -248,6 +248,8 @@ int launch_build_rev(acc_graph *g, bool validate);
 int launch_triangles(acc_graph *g);
 int launch_count(acc_graph *g);
 int launch_count_big(acc_graph *g);
+// Adds each upper slot and its reverse; with a partition, also the per-graph
+// totals (which launch_init zeroes).
 int launch_combine(acc_graph *g, int plus);
 int launch_graph_totals(acc_graph *g, int plus);
 // Slot-parallel over the sources in d_cnt (accprep::slot_sources).

@@ -632,12 +632,9 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
     ACC_TRY(accfwd::launch_count_big(g));
     ACC_TRY(record(g, timing, "k_fwd_big"));
   }
+  // for a partitioned graph the combine also reduces the per-graph totals
   ACC_TRY(accfwd::launch_combine(g, (flags & ACC_CN_RAW) ? 0 : 2));
   ACC_TRY(record(g, timing, "k_combine"));
-  if (g->num_parts) {
-    ACC_TRY(accfwd::launch_graph_totals(g, (flags & ACC_CN_RAW) ? 0 : 2));
-    ACC_TRY(record(g, timing, "k_graph_totals"));
-  }
   g->totals_valid = g->num_parts != 0;
   g->have_result = true;
   g->cnt_plus2 = (flags & ACC_CN_RAW) == 0;

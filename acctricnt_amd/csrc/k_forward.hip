@@ -45,14 +45,18 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
+// Zeroes the counts, the triangle sum and the per-graph totals of a
+// partitioned graph (parts of them; gtot may be NULL when parts is 0).
 __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_t m,
-                                              unsigned long long *__restrict__ sum) {
+                                              unsigned long long *__restrict__ sum,
+                                              unsigned long long *__restrict__ gtot, uint32_t parts) {
   const uint64_t tid = blockIdx.x * 256ull + threadIdx.x;
   const uint64_t stride = gridDim.x * 256ull;
   const uint64_t m4 = m / 4;
   const int4 z = make_int4(0, 0, 0, 0);
   for (uint64_t i = tid; i < m4; i += stride) reinterpret_cast<int4 *>(cnt)[i] = z;
   for (uint64_t i = m4 * 4 + tid; i < m; i += stride) cnt[i] = 0;
+  for (uint64_t k = tid; k < parts; k += stride) gtot[k] = 0;
   if (tid == 0) *sum = 0;
 }
 
@@ -734,6 +738,60 @@ __global__ __launch_bounds__(256) void k_graph_totals(const uint32_t *__restrict
   }
 }
 
+// Slots per workgroup of k_combine_parts: small enough that the grid is many
+// times the resident workgroups (a short tail), large enough that the
+// boundary search and the per-piece atomics stay negligible.
+constexpr uint32_t kCombSpan = 1u << 12;
+
+// k_combine for a partitioned graph, with its per-graph totals: workgroup w
+// takes the slots [w * kCombSpan, (w + 1) * kCombSpan), split at graph
+// boundaries (a reverse slot lies in its upper slot's graph), combines every
+// upper slot of a piece and adds twice the piece's raw edge totals (both
+// slots of every edge: 6 T_k in all) to its graph's total.
+__global__ __launch_bounds__(256) void k_combine_parts(int32_t *__restrict__ cnt, const uint32_t *__restrict__ rev,
+                                                       uint32_t m, int plus, unsigned long long *__restrict__ sum,
+                                                       const uint32_t *__restrict__ gsoff, uint32_t parts,
+                                                       unsigned long long *__restrict__ gtot) {
+  __shared__ unsigned long long part[4];
+  uint32_t s0 = blockIdx.x * kCombSpan;
+  const uint32_t s1 = m - s0 < kCombSpan ? m : s0 + kCombSpan;
+  // k: the last graph with gsoff[k] <= s0
+  uint32_t lo = 0, hi = parts;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (gsoff[mid] <= s0) lo = mid; else hi = mid;
+  }
+  uint32_t k = __builtin_amdgcn_readfirstlane(lo);
+  unsigned long long all = 0;
+  while (s0 < s1) {
+    const uint32_t ge = __builtin_amdgcn_readfirstlane(gsoff[k + 1]);
+    const uint32_t e = ge < s1 ? ge : s1;
+    unsigned long long s = 0;
+    for (uint32_t i = s0 + threadIdx.x; i < e; i += 256) {
+      const uint32_t p = rev[i];
+      if (p < m) {
+        int c = cnt[i] + cnt[p];
+        s += static_cast<unsigned long long>(c);
+        c += plus;
+        cnt[i] = c;
+        cnt[p] = c;
+      }
+    }
+    s = wave_sum_u64(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+      if (t) atomicAdd(&gtot[k], 2 * t);
+      all += t;
+    }
+    __syncthreads();
+    s0 = e;
+    k++;
+  }
+  if (threadIdx.x == 0 && all) atomicAdd(sum, all);
+}
+
 // Flags: 1 id out of range, 2 self-loop, 4 unsorted (duplicates included).
 // Symmetry is checked by k_build_rev.  The source of slot i is src[i]
 // (off[src[i]] <= i < off[src[i] + 1]).
@@ -769,7 +827,7 @@ namespace accfwd {
 
 int launch_init(acc_graph *g) {
   hipLaunchKernelGGL(k_init, dim3(grid_for(g->m / 4 + 1, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream,
-                     g->d_cnt, g->m, g->d_sum);
+                     g->d_cnt, g->m, g->d_sum, g->num_parts ? g->d_gtot : nullptr, g->num_parts);
   ACC_HIP_TRY(hipGetLastError());
   return ACC_OK;
 }
@@ -863,6 +921,15 @@ int launch_triangles(acc_graph *g) {
 }
 
 int launch_combine(acc_graph *g, int plus) {
+  if (g->num_parts) {
+    if (g->m) {
+      const uint32_t blocks = static_cast<uint32_t>((g->m + kCombSpan - 1) / kCombSpan);
+      hipLaunchKernelGGL(k_combine_parts, dim3(blocks), dim3(256), 0, g->ctx->stream, g->d_cnt, g->d_rev,
+                         static_cast<uint32_t>(g->m), plus, g->d_sum, g->d_gsoff, g->num_parts, g->d_gtot);
+    }
+    ACC_HIP_TRY(hipGetLastError());
+    return ACC_OK;
+  }
   hipLaunchKernelGGL(k_combine, dim3(grid_for(g->m, g->ctx->num_cus)), dim3(256), 0, g->ctx->stream, g->d_cnt,
                      g->d_rev, g->m, plus, g->d_sum);
   ACC_HIP_TRY(hipGetLastError());
