@@ -508,6 +508,8 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
                      uint32_t flags, acc_graph **out) {
   if (!ctx || !out || !node_off || (m && !edge_dst)) return acc_set_error(ACC_EINVAL, "null argument");
   *out = nullptr;
+  if (flags & ~(ACC_UPLOAD_NO_VALIDATE | ACC_UPLOAD_RELABEL))
+    return acc_set_error(ACC_EINVAL, "unknown upload flags " + std::to_string(flags));
   if (m >= (1ULL << 32)) return acc_set_error(ACC_EINVAL, "m must be below 2^32 (uint32 node_off)");
   if (n >= (1u << 31)) return acc_set_error(ACC_EINVAL, "n must be below 2^31 (int32 ids)");
   if (node_off[0] != 0 || node_off[n] != m) return acc_set_error(ACC_EFORMAT, "node_off[0] != 0 or node_off[n] != m");

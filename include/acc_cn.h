@@ -48,7 +48,8 @@ typedef struct acc_graph acc_graph;
 #define ACC_CN_TIMING 0x2u  /* record per-kernel HIP events; read them with acc_cn_timings */
 
 /* Upload flags. */
-#define ACC_UPLOAD_NO_VALIDATE 0x1u /* skip the host CSR validation (trusted input) */
+#define ACC_UPLOAD_NO_VALIDATE 0x1u /* skip the CSR validation (trusted input: ids in range,
+                                       lists sorted without self-loops, symmetric) */
 #define ACC_UPLOAD_RELABEL 0x2u     /* relabel on the device by degree, descending, ties by
                                        ascending id; implies validation.  See acc_graph_download;
                                        acc_cn_timings then gives the relabel's device time per step
@@ -68,7 +69,8 @@ const char *acc_last_error(void);
  * struct Graph, scan-query-cuda/util/graph.h:22-61) into HBM, validates it as
  * Graph::CheckInputGraph does (scan-query-cuda/util/graph.cu:196-214) plus
  * symmetry and id range (ACC_EFORMAT on failure, instead of exit(1)), and
- * prepares the per-graph work plans. */
+ * prepares the per-graph work plans.  Flags other than the ACC_UPLOAD_* ones
+ * are rejected (ACC_EINVAL). */
 int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_off,
                      const int32_t *edge_dst, uint32_t flags, acc_graph **out);
 void acc_graph_release(acc_graph *g);

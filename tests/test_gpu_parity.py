@@ -237,6 +237,21 @@ def test_malformed_upload_rejected(ctx, case, relabel):
     assert want in msg, msg
 
 
+def test_unknown_upload_flags_rejected(ctx):
+    """Upload flags other than NO_VALIDATE and RELABEL are refused (ACC_EINVAL)
+    before anything is allocated."""
+    from acctricnt_amd import _ffi
+
+    g = BF.complete(5)
+    off = np.ascontiguousarray(g.node_off, np.uint32)
+    adj = np.ascontiguousarray(g.edge_dst, np.int32)
+    h = C.c_void_p()
+    rc = ctx._lib.acc_graph_upload(ctx.handle, g.n, g.m, _ffi.ptr(off, C.c_uint32), _ffi.ptr(adj, C.c_int32),
+                                   0x4, C.byref(h))
+    assert rc == -1 and not h.value
+    assert "unknown upload flags" in ctx._lib.acc_last_error().decode()
+
+
 def test_paths_agree_rmat20(ctx):
     """R-MAT scale 20: the forward, bitmap and merge paths give identical counts,
     with T equal to the oracle's triangle count."""
