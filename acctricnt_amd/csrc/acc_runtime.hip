@@ -911,14 +911,25 @@ int acc_graph_set_partition(acc_graph *g, uint32_t num_graphs, const uint32_t *v
     for (uint32_t v = vertex_off[k]; v < vertex_off[k + 1]; v++)
       if (static_cast<uint32_t>(g->h_lo[v]) < vertex_off[k])
         return acc_set_error(ACC_EINVAL, "an edge joins two graphs of the partition");
-  ACC_TRY(dev_alloc(g, &g->d_gvoff, static_cast<uint64_t>(num_graphs) + 1));
-  ACC_TRY(dev_alloc(g, &g->d_gsoff, static_cast<uint64_t>(num_graphs) + 1));
-  ACC_TRY(dev_alloc(g, &g->d_gtot, num_graphs));
-  ACC_HIP_TRY(hipMemcpyAsync(g->d_gvoff, vertex_off, sizeof(uint32_t) * (num_graphs + 1ULL), hipMemcpyHostToDevice,
-                             g->ctx->stream));
-  ACC_HIP_TRY(hipMemcpyAsync(g->d_gsoff, gsoff.data(), sizeof(uint32_t) * (num_graphs + 1ULL), hipMemcpyHostToDevice,
-                             g->ctx->stream));
-  ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  // on any failure the partition arrays are freed again and none is set
+  auto fail = [&](int rc) {
+    (void)hipStreamSynchronize(g->ctx->stream);
+    dev_free(g, &g->d_gvoff, num_graphs + 1ULL);
+    dev_free(g, &g->d_gsoff, num_graphs + 1ULL);
+    dev_free(g, &g->d_gtot, num_graphs);
+    return rc;
+  };
+  int rc = dev_alloc(g, &g->d_gvoff, static_cast<uint64_t>(num_graphs) + 1);
+  if (!rc) rc = dev_alloc(g, &g->d_gsoff, static_cast<uint64_t>(num_graphs) + 1);
+  if (!rc) rc = dev_alloc(g, &g->d_gtot, num_graphs);
+  if (rc) return fail(rc);
+  hipError_t e = hipMemcpyAsync(g->d_gvoff, vertex_off, sizeof(uint32_t) * (num_graphs + 1ULL),
+                                hipMemcpyHostToDevice, g->ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(g->d_gsoff, gsoff.data(), sizeof(uint32_t) * (num_graphs + 1ULL), hipMemcpyHostToDevice,
+                       g->ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g->ctx->stream);
+  if (e != hipSuccess) return fail(acc_set_error(ACC_EDEVICE, std::string("partition: ") + hipGetErrorString(e)));
   g->num_parts = num_graphs;
   return ACC_OK;
 }
