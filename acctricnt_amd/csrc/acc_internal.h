@@ -268,8 +268,12 @@ int launch_source_work(acc_graph *g, unsigned long long *work);
 // free until the reverse slots are built and the first pass runs.
 // relabel_degree, given those sources, rewrites d_off and d_adj in place by
 // degree (d_rev is its scratch) and keeps the old id of every new vertex in
-// d_old; the sources must be recomputed after it.
+// d_old; the sources must be recomputed after it.  reverse_slots, given the
+// sources and d_hs, fills d_rev (d_cnt is its scratch) and returns the number
+// of upper slots; with validate, an upper slot without its reverse sets flag
+// 8 in d_flags, and a count other than m / 2 leaves d_rev unbuilt.
 namespace accprep {
 int slot_sources(acc_graph *g);
 int relabel_degree(acc_graph *g);
+int reverse_slots(acc_graph *g, bool validate, uint64_t *upper);
 }  // namespace accprep

@@ -556,14 +556,13 @@ int acc_graph_upload(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *node_
     if (hf[0] & 4) return bail(acc_set_error(ACC_EFORMAT, "Edges not sorted in increasing id order"));
   }
   if (relabel && ((rc = accprep::relabel_degree(g)) || (rc = accprep::slot_sources(g)))) return bail(rc);
-  if ((rc = accfwd::launch_build_rev(g, validate))) return bail(rc);
+  if ((rc = load_vertex_meta(g, relabel ? nullptr : node_off))) return bail(rc);
+  uint64_t upper = 0;
+  if ((rc = accprep::reverse_slots(g, validate, &upper))) return bail(rc);
   if (validate) {
     if ((rc = read_flags())) return bail(rc);
-    uint64_t upper = 0;
-    std::memcpy(&upper, hf + 2, sizeof(upper));
     if ((hf[0] & 8) || 2 * upper != m) return bail(acc_set_error(ACC_EFORMAT, "CSR not symmetric"));
   }
-  if ((rc = load_vertex_meta(g, relabel ? nullptr : node_off))) return bail(rc);
   if ((rc = upload_plans(g))) return bail(rc);
   *out = g;
   return ACC_OK;

@@ -11,7 +11,7 @@
 //          own slot.
 // Upper slots therefore receive only plain stores and lower slots only
 // atomics.  The reverse slot of every upper slot (b,c), the position of b in
-// N(c), is computed once per resident graph (k_build_rev); it bounds the
+// N(c), is computed once per resident graph (at upload); it bounds the
 // probe of N(c) to exactly |N(c) ∩ [0, b)| elements, which lets a wave or a
 // workgroup flatten all probes of a source into one element range with no
 // idle lane, and k_combine uses it to add the two halves and write
@@ -116,7 +116,9 @@ __global__ __launch_bounds__(256) void k_vertex_meta(const uint32_t *__restrict_
   }
 }
 
-// Reverse slots, computed once per resident graph: rev[j] = position of u in
+// Reverse slots by binary search, used at upload only for a CSR trusted
+// without validation whose upper slots outnumber its lower ones (the sort
+// of the upper slots by target does the rest): rev[j] = position of u in
 // N(v) for every upper slot j = (u, v), u < v (binary search of N(v)), ~0
 // for lower slots and for an upper slot whose reverse is absent (possible
 // only in an unvalidated, asymmetric CSR).  One thread per slot, the source
@@ -793,7 +795,7 @@ __global__ __launch_bounds__(256) void k_combine_parts(int32_t *__restrict__ cnt
 }
 
 // Flags: 1 id out of range, 2 self-loop, 4 unsorted (duplicates included).
-// Symmetry is checked by k_build_rev.  The source of slot i is src[i]
+// Symmetry is checked by the reverse-slot build.  The source of slot i is src[i]
 // (off[src[i]] <= i < off[src[i] + 1]).
 __global__ __launch_bounds__(256) void k_validate(const uint32_t *__restrict__ off,
                                                   const int32_t *__restrict__ adj,

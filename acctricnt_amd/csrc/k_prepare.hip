@@ -19,6 +19,15 @@
 // offsets (an exclusive scan of the sorted degrees), one thread per old slot
 // writing its neighbor's new id at the new position (m), and a segmented
 // radix sort of the new lists over ceil(log2 n) key bits (m).
+//
+// Reverse slots: the upper slots (v, c), c > v, taken in CSR order and
+// sorted stably by target c, come out grouped by c with v ascending inside
+// each group, which is the order of c's lower slots; so the k-th sorted
+// entry's reverse is c's lower slot k - L[c], where L is the exclusive scan
+// of the lower-slot counts.  This replaces the reference's per-edge binary
+// search for the reverse offset (scan-query-cuda/experimental_scan_xp_multigpu_multipass.cu:6-23,
+// scan-query/scan_xp.cpp:88-92) by HBM-streaming passes plus one scattered
+// write per upper slot.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -74,6 +83,61 @@ __global__ __launch_bounds__(256) void k_rl_fill(const uint32_t *__restrict__ of
   }
 }
 
+// low[v] = number of lower slots of v (neighbors below v), low[n] = 0.
+__global__ __launch_bounds__(256) void k_rev_low(const uint32_t *__restrict__ off, const uint32_t *__restrict__ hs,
+                                                 uint32_t n, uint32_t *__restrict__ low) {
+  for (uint32_t v = blockIdx.x * 256u + threadIdx.x; v <= n; v += gridDim.x * 256u)
+    low[v] = v < n ? hs[v] - off[v] : 0;
+}
+
+// Upper slot j of source v (j >= hs[v]) is the (j - L[v + 1])-th upper slot
+// in CSR order (the slots before it, less the lower slots of vertices up to
+// v); it is written there with its target as the key.
+__global__ __launch_bounds__(256) void k_rev_keys(const int32_t *__restrict__ adj, const uint32_t *__restrict__ src,
+                                                  const uint32_t *__restrict__ hs, const uint32_t *__restrict__ L,
+                                                  uint64_t m, uint32_t n, uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ vals) {
+  for (uint64_t j = blockIdx.x * 256ull + threadIdx.x; j < m; j += gridDim.x * 256ull) {
+    const uint32_t v = src[j];
+    if (j >= hs[v]) {
+      const uint32_t k = static_cast<uint32_t>(j) - L[v + 1];
+      const int32_t c = adj[j];
+      // ids are checked unless the caller vouches for them; an id out of
+      // range still keeps every access below in bounds
+      keys[k] = c >= 0 && static_cast<uint32_t>(c) < n ? static_cast<uint32_t>(c) : n - 1;
+      vals[k] = static_cast<uint32_t>(j);
+    }
+  }
+}
+
+// Sorted entry k = (target c, upper slot j): its reverse is c's lower slot
+// t = off[c] + k - L[c], which must lie among c's lower slots and hold the
+// source of j.  When every entry passes and the upper slots are half of all
+// slots, the entries map one-to-one onto the lower slots, so the CSR is
+// symmetric; a failing entry sets flag 8 and leaves rev[j] at ~0.
+__global__ __launch_bounds__(256) void k_rev_fill(const uint32_t *__restrict__ keys,
+                                                  const uint32_t *__restrict__ vals, uint32_t count,
+                                                  const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                  const uint32_t *__restrict__ L, uint32_t n,
+                                                  uint32_t *__restrict__ rev, int32_t *__restrict__ flags) {
+  int32_t f = 0;
+  for (uint32_t k = blockIdx.x * 256u + threadIdx.x; k < count; k += gridDim.x * 256u) {
+    const uint32_t c = keys[k], j = vals[k];
+    const uint32_t r = k - L[c];
+    bool ok = false;
+    if (r < L[c + 1] - L[c]) {
+      const uint32_t t = off[c] + r;
+      const int32_t v = adj[t];
+      if (v >= 0 && static_cast<uint32_t>(v) < n && off[v] <= j && j < off[v + 1]) {
+        rev[j] = t;
+        ok = true;
+      }
+    }
+    if (!ok) f = 8;
+  }
+  if (flags && f) atomicOr(flags, f);
+}
+
 unsigned rl_grid(uint64_t items, int cus) {
   const uint64_t want = (items + 255) / 256;
   const uint64_t cap = static_cast<uint64_t>(cus > 0 ? cus : 256) * 8;
@@ -107,6 +171,84 @@ int slot_sources(acc_graph *g) {
   if (e != hipSuccess)
     return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
                          std::string("slot sources: ") + hipGetErrorString(e));
+  return ACC_OK;
+}
+
+int reverse_slots(acc_graph *g, bool validate, uint64_t *upper) {
+  const uint32_t n = g->n;
+  const uint64_t m = g->m;
+  *upper = 0;
+  if (!m) return ACC_OK;
+  hipStream_t st = g->ctx->stream;
+  const int cus = g->ctx->num_cus;
+  uint32_t *low = nullptr, *L = nullptr;
+  void *temp = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (void *p : {static_cast<void *>(low), static_cast<void *>(L), temp})
+      if (p) (void)hipFree(p);
+  };
+  auto fail = [&](const char *what, hipError_t e) {
+    cleanup();
+    return acc_set_error(e == hipErrorOutOfMemory ? ACC_ENOMEM : ACC_EDEVICE,
+                         std::string("reverse slots ") + what + ": " + hipGetErrorString(e));
+  };
+  const size_t nb1 = sizeof(uint32_t) * (static_cast<size_t>(n) + 1);
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&low), nb1);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&L), nb1);
+  size_t t_scan = 0;
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, low, L, static_cast<int>(n) + 1, st);
+  if (e == hipSuccess) e = hipMalloc(&temp, t_scan ? t_scan : 1);
+  if (e != hipSuccess) return fail("allocation", e);
+  hipLaunchKernelGGL(k_rev_low, dim3(rl_grid(static_cast<uint64_t>(n) + 1, cus)), dim3(256), 0, st, g->d_off,
+                     g->d_hs, n, low);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(temp, t_scan, low, L, static_cast<int>(n) + 1, st);
+  uint32_t lower = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&lower, L + n, sizeof(lower), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail("lower-slot offsets", e);
+  const uint64_t count = m - lower;
+  *upper = count;
+  if (validate && 2 * count != m) {  // not symmetric: the caller rejects the graph
+    cleanup();
+    return ACC_OK;
+  }
+  // The sort's input pairs go into the reverse-slot array and its output into
+  // the count array (slot sources, read by k_rev_keys before the sort writes).
+  // A graph too lopsided for that, trusted without validation, takes the
+  // per-slot binary search instead.
+  if (2 * count > m + 4 || count > static_cast<uint64_t>(INT32_MAX)) {
+    cleanup();
+    return accfwd::launch_build_rev(g, false);
+  }
+  uint32_t *keys_in = g->d_rev, *vals_in = g->d_rev + count;
+  uint32_t *keys_out = reinterpret_cast<uint32_t *>(g->d_cnt), *vals_out = keys_out + count;
+  const int bits = bit_width(n - 1);
+  size_t t_sort = 0;
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, keys_in, keys_out, vals_in, vals_out,
+                                         static_cast<int>(count), 0, bits, st);
+  if (e == hipSuccess && t_sort > t_scan) {
+    (void)hipFree(temp);
+    temp = nullptr;
+    e = hipMalloc(&temp, t_sort);
+  }
+  if (e != hipSuccess) return fail("sort storage", e);
+  hipLaunchKernelGGL(k_rev_keys, dim3(rl_grid(m, cus)), dim3(256), 0, st, g->d_adj,
+                     reinterpret_cast<const uint32_t *>(g->d_cnt), g->d_hs, L, m, n, keys_in, vals_in);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(temp, t_sort, keys_in, keys_out, vals_in, vals_out,
+                                           static_cast<int>(count), 0, bits, st);
+  if (e == hipSuccess) e = hipMemsetAsync(g->d_rev, 0xff, sizeof(uint32_t) * (m + 4), st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rev_fill, dim3(rl_grid(count, cus)), dim3(256), 0, st, keys_out, vals_out,
+                       static_cast<uint32_t>(count), g->d_off, g->d_adj, L, n, g->d_rev,
+                       validate ? g->d_flags : nullptr);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) return fail("build", e);
+  cleanup();
   return ACC_OK;
 }
 

@@ -218,6 +218,10 @@ MALFORMED = {
     "range": ([0, 2, 4, 6], [1, 5, 0, 2, 0, 1]),
     "asym_upper": ([0, 2, 3, 3], [1, 2, 0]),  # (0,2) has no reverse
     "asym_lower": ([0, 1, 2, 3], [1, 0, 0]),  # (2,0) has no reverse; every upper slot has one
+    # as many upper as lower slots, but (2,3) and (3,1) have no reverse
+    "asym_balanced": ([0, 1, 2, 3, 4], [1, 0, 3, 1]),
+    # the same beside matched edges: (0,1), (0,2) are symmetric, (2,3) / (3,1) are not
+    "asym_balanced_long": ([0, 2, 3, 5, 6], [1, 2, 0, 0, 3, 1]),
 }
 
 
@@ -233,7 +237,8 @@ def test_malformed_upload_rejected(ctx, case, relabel):
     assert e.value.code == -2
     msg = str(e.value)
     want = {"selfloop": "Self loop", "unsorted": "not sorted", "duplicate": "not sorted",
-            "range": "out of range", "asym_upper": "not symmetric", "asym_lower": "not symmetric"}[case]
+            "range": "out of range", "asym_upper": "not symmetric", "asym_lower": "not symmetric",
+            "asym_balanced": "not symmetric", "asym_balanced_long": "not symmetric"}[case]
     assert want in msg, msg
 
 
