@@ -554,27 +554,31 @@ def main():
     other_paths = None if args.no_paths or spec["kind"] == "batch" else bench_other_paths(rg, b_alg, cnt)
     rg.release()
     # end to end from host memory: upload (copy, validation, reverse slots,
-    # plans), one pass, counts and total back in host memory
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    rg = ctx.upload(g)
-    if "vertex_off" in g.meta:
-        rg.set_partition(g.meta["vertex_off"])
-    for k, v in params.items():
-        rg.set_param(k, v)
-    t1 = time.perf_counter()
-    rg.compute(args.path)
-    ctx.synchronize()
-    t2 = time.perf_counter()
-    cnt_e2e, tri_e2e = rg.download()
-    t3 = time.perf_counter()
-    e2e_ms = (t3 - t0) * 1e3
-    e2e_parts = {"upload_ms": (t1 - t0) * 1e3, "pass_ms": (t2 - t1) * 1e3, "download_ms": (t3 - t2) * 1e3}
-    e2e_same = bool(tri_e2e == tri and np.array_equal(cnt_e2e, cnt))
-    del cnt_e2e
+    # plans), one pass, counts and total back in host memory; the best of
+    # three runs, since the host-side copies vary between runs
+    e2e_ms, e2e_parts, e2e_same = None, None, True
+    for _ in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rg = ctx.upload(g)
+        if "vertex_off" in g.meta:
+            rg.set_partition(g.meta["vertex_off"])
+        for k, v in params.items():
+            rg.set_param(k, v)
+        t1 = time.perf_counter()
+        rg.compute(args.path)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        cnt_e2e, tri_e2e = rg.download()
+        t3 = time.perf_counter()
+        rg.release()
+        e2e_same = e2e_same and bool(tri_e2e == tri and np.array_equal(cnt_e2e, cnt))
+        del cnt_e2e
+        if e2e_ms is None or (t3 - t0) * 1e3 < e2e_ms:
+            e2e_ms = (t3 - t0) * 1e3
+            e2e_parts = {"upload_ms": (t1 - t0) * 1e3, "pass_ms": (t2 - t1) * 1e3, "download_ms": (t3 - t2) * 1e3}
     rev = g.reverse_slots()
     consistent = bool(np.array_equal(cnt, cnt[rev]) and int((cnt.astype(np.int64) - 2).sum()) == 6 * tri)
-    rg.release()
     relabel = None if args.no_relabel else bench_relabel(ctx, spec, g)
     shards = None if args.no_shards or spec["kind"] == "batch" else bench_shards(ctx, g, args.shards, params)
     batch = None if args.no_batch else bench_batch(ctx, args.steps, args.warmup, args.batch_graphs)
@@ -618,7 +622,7 @@ def main():
                    "end_to_end_ms": e2e_ms, "end_to_end_parts": e2e_parts,
                    "end_to_end_slots_per_s": g.m / (e2e_ms * 1e-3),
                    "end_to_end_note": "host arrays (pageable) to counts in host memory: upload with validation "
-                                       "and reverse slots, one pass, download; results equal to the timed "
+                                       "and reverse slots, one pass, download (best of three runs); results equal to the timed "
                                        f"passes: {e2e_same}"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(dom, args.workload),
