@@ -242,6 +242,26 @@ def test_malformed_upload_rejected(ctx, case, relabel):
     assert want in msg, msg
 
 
+@pytest.mark.parametrize("case", ["star_out", "asym_balanced"])
+def test_trusted_asymmetric_upload(ctx, case):
+    """An asymmetric CSR uploaded without validation (the caller vouches for it)
+    still uploads and counts within its arrays: with more upper than lower slots
+    the reverse slots come from the per-slot binary search, otherwise from the
+    sort.  Neither graph has a triangle: an edge listed both ways gets 2 in
+    both slots, a slot whose reverse is absent keeps its raw count, 0."""
+    if case == "star_out":  # vertex 0 lists 1..10, no list lists 0
+        off = np.array([0] + [10] * 11, np.uint32)
+        adj = np.arange(1, 11, dtype=np.int32)
+        want = np.zeros(10, np.int32)
+    else:  # (0,1) listed both ways, (2,3) and (3,1) one way only
+        off, adj = (np.array(x, t) for x, t in zip(MALFORMED[case], (np.uint32, np.int32)))
+        want = np.array([2, 2, 0, 0], np.int32)
+    g = G.CSRGraph(off, adj)
+    with ctx.upload(g, validate=False) as rg:
+        cnt, t = rg.count_all_edges("forward")
+    assert np.array_equal(cnt, want) and t == 0
+
+
 def test_unknown_upload_flags_rejected(ctx):
     """Upload flags other than NO_VALIDATE and RELABEL are refused (ACC_EINVAL)
     before anything is allocated."""
