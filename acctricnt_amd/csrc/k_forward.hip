@@ -522,6 +522,12 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
   }
   const uint32_t P = wave_excl_scan(len, lane);
   const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
+  // per probe, for one lane exchange each per element: element e of the
+  // probe is slot cbP + e (mod 2^32), and its source's table start (< 512),
+  // table bits (<= 9) and first counter (<= 256) packed into one word
+  static_assert(accfwd::kWaveTableBits <= 9 && accfwd::kWaveCap <= 512, "the packing below needs 9 + 4 + 10 bits");
+  const uint32_t cbP = cb - P;
+  const uint32_t meta = p_tb | (static_cast<uint32_t>(p_bits) << 9) | (p_nl << 13);
   wave_lds_sync();
   uint32_t found = 0;
   for (uint32_t e0 = 0; e0 < T; e0 += 64 * U) {
@@ -533,14 +539,15 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
       const uint32_t e = e0 + 64 * q + lane;
       const int k = lane_search(P, e);
       kq[q] = k;
-      idx[q] = static_cast<uint32_t>(__shfl(cb, k)) + (e - static_cast<uint32_t>(__shfl(P, k)));
+      idx[q] = static_cast<uint32_t>(__shfl(cbP, k)) + e;
       a[q] = e < T ? adj[idx[q]] : -1;
     }
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      const uint32_t tb = __shfl(p_tb, kq[q]);
-      const int tbits = __shfl(p_bits, kq[q]);
-      const uint32_t nlb = __shfl(p_nl, kq[q]);
+      const uint32_t pm = static_cast<uint32_t>(__shfl(meta, kq[q]));
+      const uint32_t tb = pm & 511u;
+      const int tbits = static_cast<int>((pm >> 9) & 15u);
+      const uint32_t nlb = pm >> 13;
       bool hit = false;
       if (a[q] >= 0 && tbits > 0) {
         const int pos = lds_find(keys + tb, kpos + tb, tbits, (1u << tbits) - 1, a[q]);
