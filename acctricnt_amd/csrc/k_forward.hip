@@ -281,7 +281,8 @@ constexpr size_t kCountLdsBytes = kBlockLdsBytes > kWaveLdsBytes ? kBlockLdsByte
 
 // Workgroup-flattened probing of a chunk's nup <= kChunk upper slots: s_P
 // (kChunk entries, T beyond nup) holds the exclusive prefix of the probe
-// lengths and s_cb the starts of the N(c_k).  Wave w takes the rounds
+// lengths and s_cb the start of each N(c_k) less its prefix, so that element
+// e of probe k is slot s_cb[k] + e (mod 2^32).  Wave w takes the rounds
 // w, w + kWaves, ... of 64 * U consecutive elements, so the chunk's work is
 // split evenly over the waves whatever the list lengths; k(e) is a 9-step
 // search of s_P.  Round indices are wave-uniform.
@@ -304,7 +305,7 @@ __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__
       for (int st = 256; st; st >>= 1)
         if (s_P[k + st] <= e) k += st;
       kq[q] = k;
-      idx[q] = s_cb[k] + (e - s_P[k]);
+      idx[q] = s_cb[k] + e;
       a[q] = e < T ? adj[idx[q]] : -1;
     }
 #pragma unroll
@@ -361,13 +362,12 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
   if (kSlots)
     for (uint32_t i = tid; i < nctr; i += kWG) ctr[i] = 0;
   // probe lengths L_t = |N(c_t) ∩ [0, b)| = rev - start of N(c_t), and their prefix
-  uint32_t len = 0;
+  uint32_t len = 0, cb = 0;
   if (static_cast<uint32_t>(tid) < nup) {
     const int32_t c = adj[u.j0 + tid];
-    const uint32_t cb = off[c];
+    cb = off[c];
     const uint32_t p = rev[u.j0 + tid];
     len = p != ~0u ? p - cb : 0;
-    s_cb[tid] = cb;
   }
   s_reg[tid] = 0;
   const uint32_t excl = wave_excl_scan(len, lane);
@@ -381,6 +381,7 @@ __device__ __forceinline__ void fwd_block_unit(const uint32_t *__restrict__ off,
     T += ws;
   }
   s_P[tid] = static_cast<uint32_t>(tid) < nup ? before + excl : T;
+  if (static_cast<uint32_t>(tid) < nup) s_cb[tid] = cb - (before + excl);
   if (bitmap) {
     uint32_t *bm = reinterpret_cast<uint32_t *>(keys);
     for (uint32_t i = tid; i < nl; i += kWG) {
