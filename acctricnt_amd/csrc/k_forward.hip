@@ -240,6 +240,15 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
   return x - v;
 }
 
+// Atomic add on an LDS word through an LDS-qualified pointer, so that it
+// is always a DS instruction: with two plain atomicAdds on either side of a
+// branch (one on LDS, one on global memory) the compiler selects between
+// the two pointers and issues one FLAT atomic for both.
+typedef __attribute__((address_space(3))) int32_t lds_int32;
+__device__ __forceinline__ void lds_atomic_add(int32_t *p, int v) {
+  __hip_atomic_fetch_add((lds_int32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Records one triangle {a, b, c} found at element slot idx of N(c): the
 // (c,a) slot itself and the (b,a) counter of a's position pos in N<(b)
 // (LDS below ctr_cap, else an atomic on the lower slot sb+pos).
@@ -247,7 +256,7 @@ __device__ __forceinline__ void count_triangle(int32_t *cnt, int32_t *ctr, uint3
                                                uint32_t idx, int pos) {
   atomicAdd(&cnt[idx], 1);
   if (static_cast<uint32_t>(pos) < ctr_cap)
-    atomicAdd(&ctr[pos], 1);
+    lds_atomic_add(&ctr[pos], 1);
   else
     atomicAdd(&cnt[sb + pos], 1);
 }
