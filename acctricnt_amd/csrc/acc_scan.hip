@@ -734,13 +734,11 @@ int build_index(acc_graph *g) {
   hipStream_t st = g->ctx->stream;
   const int cus = g->ctx->num_cus;
   float *key_in = nullptr;
-  uint32_t *slot_in = nullptr;
   void *temp = nullptr;
   size_t temp_bytes = 0;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(st);
     if (key_in) (void)hipFree(key_in);
-    if (slot_in) (void)hipFree(slot_in);
     if (temp) (void)hipFree(temp);
   };
   auto fail = [&](const char *what, hipError_t e) {
@@ -756,26 +754,35 @@ int build_index(acc_graph *g) {
   };
   const size_t mm = m ? m : 1;
   hipError_t e = hipMalloc(reinterpret_cast<void **>(&key_in), sizeof(float) * mm);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&slot_in), sizeof(uint32_t) * mm);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_skey), sizeof(float) * mm);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_sslot), sizeof(uint32_t) * mm);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&g->d_snbr), sizeof(int32_t) * mm);
   if (e != hipSuccess) return fail("index allocation", e);
   if (m) {
+    // the sort is double-buffered (keys between key_in and d_skey, slots
+    // between d_snbr, free until the gather, and d_sslot), so its temporary
+    // storage holds no copy of the pairs
+    uint32_t *const slot_in = reinterpret_cast<uint32_t *>(g->d_snbr);
     hipLaunchKernelGGL(k_idx_key, dim3(grid_for(m, cus)), dim3(256), 0, st, g->d_off, g->d_adj, g->d_src, g->d_cnt,
                        m, g->cnt_plus2 ? 0 : 2, key_in, slot_in);
     e = hipGetLastError();
     if (e != hipSuccess) return fail("index keys", e);
-    e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(nullptr, temp_bytes, key_in, g->d_skey, slot_in,
-                                                              g->d_sslot, static_cast<int>(m),
+    hipcub::DoubleBuffer<float> kb(key_in, g->d_skey);
+    hipcub::DoubleBuffer<uint32_t> sb(slot_in, g->d_sslot);
+    e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(nullptr, temp_bytes, kb, sb, static_cast<int>(m),
                                                               static_cast<int>(g->n), g->d_off, g->d_off + 1, 0,
                                                               32, st);
     if (e == hipSuccess) e = hipMalloc(&temp, temp_bytes ? temp_bytes : 1);
+    kb = hipcub::DoubleBuffer<float>(key_in, g->d_skey);  // as before the size query
+    sb = hipcub::DoubleBuffer<uint32_t>(slot_in, g->d_sslot);
     if (e == hipSuccess)
-      e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(temp, temp_bytes, key_in, g->d_skey, slot_in,
-                                                                g->d_sslot, static_cast<int>(m),
+      e = hipcub::DeviceSegmentedRadixSort::SortPairsDescending(temp, temp_bytes, kb, sb, static_cast<int>(m),
                                                                 static_cast<int>(g->n), g->d_off, g->d_off + 1, 0,
                                                                 32, st);
+    if (e == hipSuccess && kb.Current() != g->d_skey)
+      e = hipMemcpyAsync(g->d_skey, kb.Current(), sizeof(float) * m, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && sb.Current() != g->d_sslot)
+      e = hipMemcpyAsync(g->d_sslot, sb.Current(), sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return fail("index sort", e);
     hipLaunchKernelGGL(k_idx_gather, dim3(grid_for(m, cus)), dim3(256), 0, st, g->d_adj, g->d_sslot, m, g->d_snbr);
     e = hipGetLastError();

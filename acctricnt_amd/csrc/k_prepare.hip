@@ -225,9 +225,11 @@ int reverse_slots(acc_graph *g, bool validate, uint64_t *upper) {
   uint32_t *keys_in = g->d_rev, *vals_in = g->d_rev + count;
   uint32_t *keys_out = reinterpret_cast<uint32_t *>(g->d_cnt), *vals_out = keys_out + count;
   const int bits = bit_width(n - 1);
+  // double-buffered: the sort ping-pongs between the two arrays instead of
+  // taking a copy of its input as temporary storage
+  hipcub::DoubleBuffer<uint32_t> kb(keys_in, keys_out), vb(vals_in, vals_out);
   size_t t_sort = 0;
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, keys_in, keys_out, vals_in, vals_out,
-                                         static_cast<int>(count), 0, bits, st);
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, t_sort, kb, vb, static_cast<int>(count), 0, bits, st);
   if (e == hipSuccess && t_sort > t_scan) {
     (void)hipFree(temp);
     temp = nullptr;
@@ -237,9 +239,14 @@ int reverse_slots(acc_graph *g, bool validate, uint64_t *upper) {
   hipLaunchKernelGGL(k_rev_keys, dim3(rl_grid(m, cus)), dim3(256), 0, st, g->d_adj,
                      reinterpret_cast<const uint32_t *>(g->d_cnt), g->d_hs, L, m, n, keys_in, vals_in);
   e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(temp, t_sort, keys_in, keys_out, vals_in, vals_out,
-                                           static_cast<int>(count), 0, bits, st);
+  kb = hipcub::DoubleBuffer<uint32_t>(keys_in, keys_out);  // as before the size query
+  vb = hipcub::DoubleBuffer<uint32_t>(vals_in, vals_out);
+  if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(temp, t_sort, kb, vb, static_cast<int>(count), 0, bits, st);
+  // the fill rewrites the reverse-slot array: the sorted pairs must lie in the count array
+  if (e == hipSuccess && kb.Current() != keys_out)
+    e = hipMemcpyAsync(keys_out, kb.Current(), sizeof(uint32_t) * count, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess && vb.Current() != vals_out)
+    e = hipMemcpyAsync(vals_out, vb.Current(), sizeof(uint32_t) * count, hipMemcpyDeviceToDevice, st);
   if (e == hipSuccess) e = hipMemsetAsync(g->d_rev, 0xff, sizeof(uint32_t) * (m + 4), st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_rev_fill, dim3(rl_grid(count, cus)), dim3(256), 0, st, keys_out, vals_out,
@@ -295,9 +302,13 @@ int relabel_degree(acc_graph *g) {
   e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t_sort, deg, deg_sorted, id, old_of_new,
                                                    static_cast<int>(n), 0, bits, st);
   if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, t_scan, len, new_off, static_cast<int>(n) + 1, st);
+  // the lists' sort is double-buffered between the mapped lists and the
+  // adjacency array (the old lists are dead once mapped), so its temporary
+  // storage holds no copy of the m keys
+  hipcub::DoubleBuffer<uint32_t> lb(mapped, lists);
   if (e == hipSuccess && m)
-    e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, t_seg, mapped, lists, static_cast<int>(m),
-                                                   static_cast<int>(n), new_off, new_off + 1, 0, bits, st);
+    e = hipcub::DeviceSegmentedRadixSort::SortKeys(nullptr, t_seg, lb, static_cast<int>(m), static_cast<int>(n),
+                                                   new_off, new_off + 1, 0, bits, st);
   size_t t_bytes = std::max(t_sort, std::max(t_scan, t_seg));
   if (e == hipSuccess) e = hipMalloc(&temp, t_bytes ? t_bytes : 1);
   if (e != hipSuccess) return fail("temporary storage", e);
@@ -331,9 +342,12 @@ int relabel_degree(acc_graph *g) {
                        new_off, mapped);
     e = hipGetLastError();
     if (e == hipSuccess) (void)record_event(g, "rl_fill");
+    lb = hipcub::DoubleBuffer<uint32_t>(mapped, lists);  // as before the size query
     if (e == hipSuccess)
-      e = hipcub::DeviceSegmentedRadixSort::SortKeys(temp, t_bytes, mapped, lists, static_cast<int>(m),
-                                                     static_cast<int>(n), new_off, new_off + 1, 0, bits, st);
+      e = hipcub::DeviceSegmentedRadixSort::SortKeys(temp, t_bytes, lb, static_cast<int>(m), static_cast<int>(n),
+                                                     new_off, new_off + 1, 0, bits, st);
+    if (e == hipSuccess && lb.Current() != lists)
+      e = hipMemcpyAsync(lists, lb.Current(), sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return fail("adjacency lists", e);
     (void)record_event(g, "rl_sort_lists");
   }
