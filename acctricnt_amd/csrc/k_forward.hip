@@ -764,6 +764,22 @@ __global__ __launch_bounds__(256) void k_graph_totals(const uint32_t *__restrict
 // boundary search and the per-piece atomics stay negligible.
 constexpr uint32_t kCombSpan = 1u << 12;
 
+// Span of workgroup b in a grid of N: workgroups reach the 8 XCDs
+// round-robin, so b is remapped to give each XCD runs of kXcdRun
+// consecutive spans (4 MiB of counts, the size of an XCD's L2), the runs
+// interleaved over the whole array: a small graph's reverse slots then lie
+// in one XCD's L2.  A bijection on [0, N); the tail short of a round of 8
+// runs keeps its order.  On the 512-graph mix, runs of 64 / 256 / 1024
+// spans made the whole pass 0.45 / 0.7 / 0.7 ms shorter than the plain
+// order (17.85 ms).
+constexpr uint32_t kXcds = 8, kXcdRun = 256;
+__device__ __forceinline__ uint32_t xcd_span(uint32_t b, uint32_t N) {
+  const uint32_t full = N / (kXcds * kXcdRun) * (kXcds * kXcdRun);
+  if (b >= full) return b;
+  const uint32_t x = b % kXcds, local = b / kXcds;
+  return (local / kXcdRun) * (kXcds * kXcdRun) + x * kXcdRun + local % kXcdRun;
+}
+
 // k_combine for a partitioned graph, with its per-graph totals: workgroup w
 // takes the slots [w * kCombSpan, (w + 1) * kCombSpan), split at graph
 // boundaries (a reverse slot lies in its upper slot's graph), combines every
@@ -774,7 +790,7 @@ __global__ __launch_bounds__(256) void k_combine_parts(int32_t *__restrict__ cnt
                                                        const uint32_t *__restrict__ gsoff, uint32_t parts,
                                                        unsigned long long *__restrict__ gtot) {
   __shared__ unsigned long long part[4];
-  uint32_t s0 = blockIdx.x * kCombSpan;
+  uint32_t s0 = xcd_span(blockIdx.x, gridDim.x) * kCombSpan;
   const uint32_t s1 = m - s0 < kCombSpan ? m : s0 + kCombSpan;
   // k: the last graph with gsoff[k] <= s0
   uint32_t lo = 0, hi = parts;
