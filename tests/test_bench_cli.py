@@ -1,0 +1,48 @@
+"""bench.py's command-line contract and its reading of the committed profiles
+(CPU only: nothing here initialises a GPU)."""
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run_bench(*args):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], cwd=ROOT,
+                          capture_output=True, text=True, timeout=120)
+
+
+@pytest.mark.parametrize("gpus", ["2", "8"])
+def test_more_than_one_gpu_is_refused(gpus):
+    r = run_bench("--gpus", gpus)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_help_lists_the_contract_flags():
+    r = run_bench("--help")
+    assert r.returncode == 0
+    for flag in ("--gpus", "--steps", "--warmup", "--workload"):
+        assert flag in r.stdout
+
+
+@pytest.mark.parametrize("workload", ["rmat22", "chunglu23", "batch512"])
+def test_traffic_comes_from_the_committed_profile(workload):
+    """Each bench workload's dominant kernel has a committed rocprofv3 summary whose
+    traffic line bench.py reads, and the figure equals the calibrated FETCH + WRITE the
+    same file states in GB."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    t = bench.profiled_traffic("k_fwd_count<4>", workload)
+    assert t is not None and t > 0
+    text = open(os.path.join(ROOT, "profiles", f"k_fwd_count-{workload}.md")).read()
+    fetch, write, total = map(float, re.search(
+        r"fetch ([0-9.]+) GB \+ write ([0-9.]+) GB = \*\*([0-9.]+) GB\*\*", text).groups())
+    assert abs(fetch + write - total) < 0.02
+    assert abs(t / 1e9 - total) < 0.01
+    # every profile names the command that made it, on committed files only
+    assert "rocprofv3 --kernel-trace --stats" in text and "bench.py" in text and "_work" not in text
