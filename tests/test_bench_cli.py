@@ -46,3 +46,14 @@ def test_traffic_comes_from_the_committed_profile(workload):
     assert abs(t / 1e9 - total) < 0.01
     # every profile names the command that made it, on committed files only
     assert "rocprofv3 --kernel-trace --stats" in text and "bench.py" in text and "_work" not in text
+
+
+@pytest.mark.parametrize("name", sorted(f for f in os.listdir(os.path.join(ROOT, "profiles")) if f.endswith(".md")))
+def test_every_profile_names_its_command(name):
+    """Each committed profile summary gives the rocprofv3 command that made it, running
+    committed files only (bench.py from the repository root)."""
+    text = open(os.path.join(ROOT, "profiles", name)).read()
+    cmds = [ln for ln in text.splitlines() if ln.startswith("rocprofv3 ")]
+    assert cmds, name
+    for c in cmds:
+        assert " -- python3 bench.py" in c and "_work" not in c, c
