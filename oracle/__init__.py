@@ -48,6 +48,8 @@ def lib():
     L.oracle_cn_bitmap.restype = None
     L.oracle_triangle_count.argtypes = [C.c_uint32, u32p, i32p, C.c_int]
     L.oracle_triangle_count.restype = C.c_uint64
+    L.oracle_triangle_count_bitmap.argtypes = [C.c_uint32, u32p, i32p, C.c_int]
+    L.oracle_triangle_count_bitmap.restype = C.c_uint64
     L.oracle_scan_core_fp.argtypes = [C.c_uint32, u32p, i32p, i32p, C.c_double, C.c_int, u8p, i32p, i32p]
     L.oracle_scan_core_fp.restype = None
     L.oracle_scan_core_int.argtypes = [C.c_uint32, u32p, i32p, i32p, C.c_int64, C.c_int64, C.c_int,
@@ -109,6 +111,12 @@ def cn_bitmap(g, nthreads: int = 0) -> np.ndarray:
 def triangle_count(g, nthreads: int = 0) -> int:
     off, adj = _csr(g)
     return int(lib().oracle_triangle_count(g.n, _p(off, C.c_uint32), _p(adj, C.c_int32), nthreads))
+
+
+def triangle_count_bitmap(g, nthreads: int = 0) -> int:
+    """T by the bit-vector triangle count (TriCntBitmap)."""
+    off, adj = _csr(g)
+    return int(lib().oracle_triangle_count_bitmap(g.n, _p(off, C.c_uint32), _p(adj, C.c_int32), nthreads))
 
 
 def eps_fraction(eps) -> Fraction:

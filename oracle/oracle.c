@@ -10,7 +10,10 @@
  * against scipy.sparse ((A·A)∘A and trace(A^3)/6), against brute-force set
  * intersection and a brute-force SCAN written from the definitions, and
  * against each other (merge == bitmap, sum of counts == 6T, integer
- * similarity == long-double similarity).  The reference's own build is not
+ * similarity == long-double similarity).  oracle_cn_merge_slots is compared
+ * with scipy on every u<v slot and on random subsets (only the chosen slots
+ * and their reverse slots written); both triangle counts with closed forms,
+ * scipy's trace and the golden totals.  The reference's own build is not
  * available here: its set-intersection and SCAN translation units include
  * the un-vendored sparsepp header (scan-query/set-inter/csr_set_intersection_serial.h,
  * scan-query/scan_xp.cpp:7), so no _ref build exists to compare with.
@@ -191,6 +194,33 @@ uint64_t oracle_triangle_count(uint32_t n, const uint32_t *off, const int32_t *a
       int32_t v = adj[j];
       t += (uint64_t)naive_merge(adj, off[u], end[u], off[v], end[v]);
     }
+  }
+  free(end);
+  return t;
+}
+
+uint64_t oracle_triangle_count_bitmap(uint32_t n, const uint32_t *off, const int32_t *adj, int nthreads) {
+  const int nt = threads_or_default(nthreads);
+  const uint64_t words = ((uint64_t)n + 63) / 64;
+  /* node_off_end: end of N<(u) (scan_xp_triangle_cnt.cpp:669-673) */
+  uint32_t *end = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+#pragma omp parallel for schedule(static) num_threads(nt)
+  for (int64_t u = 0; u < (int64_t)n; u++) end[u] = (uint32_t)lower_bound_i32(adj, off[u], off[u + 1], (int32_t)u);
+  uint64_t t = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : t)
+  {
+    uint64_t *bits = (uint64_t *)calloc(words ? words : 1, sizeof(uint64_t));
+#pragma omp for schedule(dynamic, 100)
+    for (int64_t u = 0; u < (int64_t)n; u++) {
+      /* set N<(u) (:683-689), probe N<(v) for every v in N<(u) (:691-698), clear (:701-705) */
+      for (uint64_t j = off[u]; j < end[u]; j++) bits[adj[j] >> 6] |= 1ULL << (adj[j] & 63);
+      for (uint64_t j = off[u]; j < end[u]; j++) {
+        const int32_t v = adj[j];
+        for (uint64_t k = off[v]; k < end[v]; k++) t += (bits[adj[k] >> 6] >> (adj[k] & 63)) & 1;
+      }
+      for (uint64_t j = off[u]; j < end[u]; j++) bits[adj[j] >> 6] = 0;
+    }
+    free(bits);
   }
   free(end);
   return t;

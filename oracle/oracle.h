@@ -49,6 +49,13 @@ void oracle_cn_bitmap(uint32_t n, uint64_t m, const uint32_t *off, const int32_t
  * |N<(u) ∩ N<(v)| where N<(x) = neighbors with id < x. */
 uint64_t oracle_triangle_count(uint32_t n, const uint32_t *off, const int32_t *adj, int nthreads);
 
+/* SCAN_XP::TriCntBitmap without BIT_VEC_INDEX
+ * (scan-query/scan_xp_triangle_cnt.cpp:655-721): per thread a bit vector of
+ * N<(u), set once per u, every N<(v), v in N<(u), probed against it
+ * (ComputeCNHashBitVec, set-inter/csr_set_intersection_serial.h:124-132),
+ * then cleared.  The same T as oracle_triangle_count. */
+uint64_t oracle_triangle_count_bitmap(uint32_t n, const uint32_t *off, const int32_t *adj, int nthreads);
+
 /* Similarity decision exactly as SCAN_XP::CheckCoreCompSimCore
  * (scan-query/scan_xp.cpp:112-141): sim = (double)((long double)cnt /
  * sqrtl((d_u+1)(d_v+1))) >= eps.  Writes sim[m] (0/1), core_count[n] and

@@ -60,6 +60,7 @@ def test_complete_graph(n):
     for cnt in (oracle.cn_merge(g), oracle.cn_bitmap(g)):
         assert (cnt == n).all()  # CN = n-2, stored +2
     assert oracle.triangle_count(g) == comb(n, 3)
+    assert oracle.triangle_count_bitmap(g) == comb(n, 3)
 
 
 @pytest.mark.parametrize("g", [BF.complete_bipartite(3, 5), BF.cycle(4), BF.cycle(9), BF.star(7)],
@@ -68,6 +69,7 @@ def test_triangle_free(g):
     assert (oracle.cn_merge(g) == 2).all()
     assert (oracle.cn_bitmap(g) == 2).all()
     assert oracle.triangle_count(g) == 0
+    assert oracle.triangle_count_bitmap(g) == 0
 
 
 @pytest.mark.parametrize("k", [5, 8, 13])
@@ -78,6 +80,7 @@ def test_wheel(k):
     hub_edge = (src == 0) | (g.edge_dst == 0)
     assert (cnt[hub_edge] == 2).all() and (cnt[~hub_edge] == 1).all()
     assert oracle.triangle_count(g) == k
+    assert oracle.triangle_count_bitmap(g) == k
 
 
 @pytest.mark.parametrize("k", [1, 4, 9])
@@ -85,6 +88,7 @@ def test_windmill(k):
     g = BF.windmill(k)
     assert (oracle.cn_merge(g) == 3).all()
     assert oracle.triangle_count(g) == k
+    assert oracle.triangle_count_bitmap(g) == k
 
 
 # ------------------------------------------------------------ golden fixtures
@@ -97,6 +101,7 @@ def test_golden_counts(name):
     assert sha(cnt) == ent["cnt_sha256"]
     assert sha(oracle.cn_bitmap(g)) == ent["cnt_sha256"]
     assert oracle.triangle_count(g) == ent["triangles"]
+    assert oracle.triangle_count_bitmap(g) == ent["triangles"]
 
 
 @pytest.mark.parametrize("name", sorted(GOLDEN))
@@ -146,7 +151,41 @@ def test_counts_vs_scipy(maker):
     ref = BF.cn_scipy(g)
     assert np.array_equal(oracle.cn_merge(g), ref)
     assert np.array_equal(oracle.cn_bitmap(g), ref)
-    assert oracle.triangle_count(g) == BF.triangles_scipy(g)
+    t = BF.triangles_scipy(g)
+    assert oracle.triangle_count(g) == t
+    assert oracle.triangle_count_bitmap(g) == t
+
+
+@pytest.mark.parametrize("maker", [
+    lambda: G.gen_rmat(12, 16, seed=1),
+    lambda: G.gen_chung_lu(5000, 2.1, 16, 1000, seed=2),
+    lambda: G.gen_rmat(11, 8, seed=3, relabel=False),
+], ids=["rmat12", "chunglu5000", "rmat11_unrelabelled"])
+def test_merge_slots_vs_scipy(maker):
+    """oracle_cn_merge_slots against scipy's ((A·A)∘A)[u,v] + 2: given every
+    u<v slot, it writes every slot of the graph; given a random subset, it
+    writes exactly those slots and their reverse slots, each equal to scipy's
+    count, and leaves every other slot untouched."""
+    g = maker()
+    ref = BF.cn_scipy(g)
+    src = g.sources()
+    up = np.nonzero(src < g.edge_dst)[0]
+    full = np.full(g.m, -1, np.int32)
+    oracle.cn_merge_slots(g, full, up, src[up])
+    assert np.array_equal(full, ref)
+    rng = np.random.default_rng(7)
+    sel = rng.choice(up, up.size // 5, replace=False)
+    part = np.full(g.m, -1, np.int32)
+    oracle.cn_merge_slots(g, part, sel, src[sel])
+    want = np.zeros(g.m, bool)
+    want[sel] = True
+    want[g.reverse_slots()[sel]] = True
+    assert np.array_equal(part >= 0, want)
+    assert np.array_equal(part[want], ref[want])
+    # a planted off-by-one in one sampled slot is caught by the same comparison
+    bad = part.copy()
+    bad[sel[0]] += 1
+    assert not np.array_equal(bad[want], ref[want])
 
 
 @pytest.mark.parametrize("seed", range(6))
