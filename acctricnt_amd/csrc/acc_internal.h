@@ -138,6 +138,8 @@ struct acc_graph {
   bool graph_plus2 = true;  // count format the captured pass writes
   // last results / timing
   bool have_result = false;
+  bool counts_share = false;  // the counts are one share's (a forward pass restricted by acc_graph_set_shard)
+  bool graph_share = false;   // the captured pass counts one share
   std::vector<hipEvent_t> events;
   std::vector<std::string> event_names;
   bool timing_valid = false;

@@ -621,6 +621,7 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
     }
     g->totals_valid = g->num_parts != 0;
     g->have_result = true;
+    g->counts_share = false;  // the per-edge paths always count the whole graph
     g->cnt_plus2 = plus == 2;
     g->timing_valid = timing;
     return ACC_OK;
@@ -638,6 +639,7 @@ int acc_cn_compute(acc_graph *g, int path, uint32_t flags) {
   ACC_TRY(record(g, timing, "k_combine"));
   g->totals_valid = g->num_parts != 0;
   g->have_result = true;
+  g->counts_share = g->shard_lo != 0 || g->shard_hi != g->n;
   g->cnt_plus2 = (flags & ACC_CN_RAW) == 0;
   g->timing_valid = timing;
   return ACC_OK;
@@ -968,6 +970,7 @@ int acc_cn_capture(acc_graph *g, int path, uint32_t flags) {
   g->graph = graph;
   ACC_HIP_TRY(hipGraphInstantiate(&g->graph_exec, graph, nullptr, nullptr, 0));
   g->graph_plus2 = !(flags & ACC_CN_RAW);
+  g->graph_share = g->counts_share;
   g->have_result = false;
   g->totals_valid = false;
   return ACC_OK;
@@ -979,6 +982,7 @@ int acc_cn_replay(acc_graph *g) {
   DeviceGuard guard(g->ctx->device);
   ACC_HIP_TRY(hipGraphLaunch(g->graph_exec, g->ctx->stream));
   g->cnt_plus2 = g->graph_plus2;
+  g->counts_share = g->graph_share;
   g->timing_valid = false;
   g->have_result = true;
   g->totals_valid = g->num_parts != 0;

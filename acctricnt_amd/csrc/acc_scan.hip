@@ -1004,7 +1004,15 @@ int acc_scan_run(acc_scan_ws *w, int64_t eps_num, int64_t eps_den, int32_t mu, u
   ACC_TRY(accscan::check_eps(eps_num, eps_den));
   acc_graph *g = w->g;
   (void)hipSetDevice(g->ctx->device);
-  if (!g->have_result) ACC_TRY(acc_cn_compute(g, ACC_PATH_AUTO, 0));
+  if (!g->have_result || g->counts_share) {
+    // SCAN consumes every slot's count: one share's counts (acc_graph_set_shard) would
+    // cluster on partial counts, so a share that the counting pass would use is refused
+    const bool share = g->shard_lo != 0 || g->shard_hi != g->n;
+    if (share && g->auto_path == ACC_PATH_FORWARD)
+      return acc_set_error(ACC_EINVAL, "SCAN needs the whole graph's counts, but the forward path is restricted "
+                                       "to one share (acc_graph_set_shard(g, 1, 0, NULL) restores the whole graph)");
+    ACC_TRY(acc_cn_compute(g, ACC_PATH_AUTO, 0));
+  }
   // order this query after the graph's counts
   ACC_HIP_TRY(hipEventRecord(w->ready, g->ctx->stream));
   ACC_HIP_TRY(hipStreamWaitEvent(w->stream, w->ready, 0));

@@ -52,6 +52,7 @@ enum QueryState { kPending = 0, kRunning = 1, kDone = 2, kFailed = 3 };
 
 struct Query {
   int32_t gid = -1;
+  uint32_t n = 0;  // the graph's vertex count, kept for results read after the graph is removed
   int64_t p = 0, q = 1;
   int32_t mu = 0;
   Clock::time_point t_submit, t_done;
@@ -263,8 +264,8 @@ void finish_batch(acc_server *s, Batch &b, hipError_t e) {
     auto it = s->queries.find(t);
     if (it == s->queries.end()) continue;
     Query &q = it->second;
-    ServedGraph &sg = s->graphs[q.gid];
-    sg.running--;
+    auto gi = s->graphs.find(q.gid);  // pinned by the running count: removal is refused meanwhile
+    if (gi != s->graphs.end()) gi->second.running--;
     if (e != hipSuccess) {
       acc_set_error(ACC_EDEVICE, std::string("SCAN batch: ") + hipGetErrorString(e));
       fail_query(s, q, ACC_EDEVICE);
@@ -579,6 +580,7 @@ int acc_server_submit(acc_server *s, int32_t graph_id, int64_t eps_num, int64_t 
   const uint64_t t = s->next_ticket++;
   Query &q = s->queries[t];
   q.gid = graph_id;
+  q.n = it->second.n;
   q.p = eps_num;
   q.q = eps_den;
   q.mu = mu;
@@ -629,7 +631,7 @@ int acc_server_result(acc_server *s, uint64_t ticket, const int8_t **label, cons
   const Slot &sl = s->slots[q.slot];
   if (label) *label = sl.h_label;
   if (cluster_id) *cluster_id = sl.h_cid;
-  if (n) *n = s->graphs[q.gid].n;
+  if (n) *n = q.n;
   if (counts)
     for (int i = 0; i < 4; i++) counts[i] = q.counts[i];
   if (latency_ms) *latency_ms = ms_between(q.t_submit, q.t_done);
@@ -650,12 +652,14 @@ int acc_server_release(acc_server *s, uint64_t ticket) {
     if (q.state == kPending) {  // a pending query is cancelled
       auto pi = std::find(s->pending.begin(), s->pending.end(), ticket);
       if (pi != s->pending.end()) s->pending.erase(pi);
-      s->graphs[q.gid].waiting--;
+      auto gi = s->graphs.find(q.gid);  // pinned by the waiting count: removal is refused meanwhile
+      if (gi != s->graphs.end()) gi->second.waiting--;
       s->st.cancelled++;
     }
     slot = q.slot;
     if (slot >= 0) {
-      n = std::min(s->graphs[q.gid].n, s->slots[slot].h_cap);
+      // q.n, not the graph's entry: the graph may have been removed since the query finished
+      n = std::min(q.n, s->slots[slot].h_cap);
       whole = q.state == kFailed;
     }
     s->queries.erase(it);

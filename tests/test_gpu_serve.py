@@ -10,6 +10,7 @@ import threading
 import numpy as np
 import pytest
 
+import bruteforce as BF
 import oracle
 from acctricnt_amd import _ffi, cn, graph as G, serve
 
@@ -219,3 +220,98 @@ def test_serve_index_ties_and_fallback(ctx):
         assert st["index_queries"] == len(cases) - 2
     assert oracle.scan(cube(), oracle.cn_merge(cube()), "0.5", 3)["core_num"] == 8
     assert oracle.scan(cube(), oracle.cn_merge(cube()), "0.51", 1)["core_num"] == 0
+
+
+def test_serve_remove_graph_with_unreleased_result(ctx):
+    """A finished query whose graph is removed before its ticket is released:
+    its result stays readable with the graph's own size, the release resets
+    every cluster id its cores held, so the next query on the same (single)
+    result slot reports -1 for all its non-cores, and the removed id stays
+    unknown."""
+    a = BF.complete(40)  # every vertex a core of one cluster at eps 0.5, mu 2
+    b = G.gen_er(3000, 4, seed=67)  # few cores at eps 0.9
+    with serve.Server(ctx, max_batch=1, streams=1, slots=1) as sv:
+        ga = sv.add_graph(a)
+        gb = sv.add_graph(b)
+        t = sv.submit(ga, "0.5", 2)
+        sv.wait(t)
+        sv.remove_graph(ga)
+        r = sv.result(t)
+        assert r["label"].shape == (a.n,) and r["core_num"] == a.n
+        assert (r["cluster_id"] == 0).all()
+        sv.release(t)
+        with pytest.raises(_ffi.AccError) as e:
+            sv.submit(ga, "0.5", 2)
+        assert e.value.code == -7
+        res = sv.query(gb, "0.9", 2)
+        check(res, b, oracle.cn_merge(b), "0.9", 2)
+
+
+def test_serve_wait_timeout(ctx, graphs):
+    """acc_server_wait returns ACC_ETIMEDOUT while a query cannot run: with one
+    result slot held by a finished, unreleased query, the next query stays
+    pending; once the slot is released it runs and its result is exact."""
+    gs, cnts = graphs
+    with serve.Server(ctx, max_batch=1, streams=1, slots=1) as sv:
+        gid = sv.add_graph(gs[0])
+        t1 = sv.submit(gid, "0.3", 5)
+        sv.wait(t1)
+        t2 = sv.submit(gid, "0.5", 2)
+        with pytest.raises(_ffi.AccError) as e:
+            sv.wait(t2, 200)
+        assert e.value.code == -8
+        assert sv.poll(t2) == 0  # ACC_QUERY_PENDING
+        check(sv.result(t1), gs[0], cnts[0], "0.3", 5)
+        sv.release(t1)
+        sv.wait(t2, 60_000)
+        check(sv.result(t2), gs[0], cnts[0], "0.5", 2)
+        sv.release(t2)
+
+
+def test_serve_configured_workload(ctx):
+    """The serving configuration: 16 resident R-MAT graphs (scales drawn
+    uniformly from 16..22 with the benchmark's seed, edge factor 16,
+    degree-relabelled), every (eps, mu) of eps in {0.1, ..., 0.9} and mu in
+    {2, 5, 10, 20} submitted at once in a shuffled order, so that many
+    batches across graphs are in flight together, all through the similarity
+    index.  Each served result equals the oracle's SCAN over the oracle's
+    counts: every (eps, mu) on the graphs of scale <= 20, a fixed third of
+    them on the larger ones; a sample also equals the same query run alone."""
+    rng = np.random.default_rng(5)
+    scales = [int(x) for x in rng.integers(16, 23, size=16)]
+    gs = [G.gen_rmat(sc, 16, seed=1000 + i) for i, sc in enumerate(scales)]
+    print(f"[serve configured] scales {scales}", flush=True)
+    epss = [f"0.{k}" for k in range(1, 10)]
+    mus = [2, 5, 10, 20]
+    combos = [(k, e, mu) for k in range(len(gs)) for e in epss for mu in mus
+              if scales[k] <= 20 or (epss.index(e) + mus.index(mu)) % 3 == 0]
+    order = np.random.default_rng(11).permutation(len(combos))
+    with serve.Server(ctx, max_batch=14, streams=4, slots=64) as sv:
+        gids = [sv.add_graph(g) for g in gs]
+        assert sv.stats()["resident_graphs"] == len(gs)
+        tickets = [(combos[i], sv.submit(gids[combos[i][0]], combos[i][1], combos[i][2])) for i in order]
+        results = {}
+        for (k, eps, mu), t in tickets:
+            sv.wait(t)
+            results[(k, eps, mu)] = sv.result(t)
+            sv.release(t)
+        st = sv.stats()
+        print(f"[serve configured] {st}", flush=True)
+        assert st["completed"] == len(combos) and st["failed"] == 0 and st["evictions"] == 0
+        assert st["index_queries"] == len(combos)
+        assert st["max_batch_seen"] > 1
+    for k in range(len(gs)):
+        cnt = oracle.cn_bitmap(gs[k])
+        mine = [c for c in combos if c[0] == k]
+        print(f"[serve configured] graph {k} scale {scales[k]}: {len(mine)} queries", flush=True)
+        for c in mine:
+            check(results[c], gs[k], cnt, c[1], c[2])
+            del results[c]
+    sample = [combos[i] for i in order[:6]]
+    for k, eps, mu in sample:
+        with ctx.upload(gs[k]) as rg, rg.scan_workspace() as ws:
+            ws.run(eps, mu)
+            ws.synchronize()
+            a = ws.results()
+        ref = oracle.scan(gs[k], oracle.cn_bitmap(gs[k]), eps, mu)
+        assert np.array_equal(a["label"], ref["label"]), (k, eps, mu)

@@ -94,3 +94,25 @@ def test_shard_arguments(ctx):
         with pytest.raises(_ffi.AccError):
             rg.set_shard(4, 4)
         assert rg.set_shard(1, 0) == (0, g.n)
+
+
+def test_scan_refuses_share_counts(ctx):
+    """SCAN never clusters on one share's counts: while the forward path is
+    restricted to a share, a query is refused; with the whole graph restored
+    it recounts and equals the oracle's SCAN."""
+    g = G.gen_rmat(12, 16, seed=97)
+    with ctx.upload(g) as rg:
+        rg.set_shard(4, 1)
+        rg.count_all_edges("forward")
+        with rg.scan_workspace() as ws:
+            with pytest.raises(_ffi.AccError) as e:
+                ws.run("0.3", 5)
+            assert e.value.code == -1
+        assert rg.set_shard(1, 0) == (0, g.n)
+        with rg.scan_workspace() as ws:
+            ws.run("0.3", 5)
+            ws.synchronize()
+            res = ws.results()
+    want = oracle.scan(g, oracle.cn_merge(g), "0.3", 5)
+    assert np.array_equal(res["label"], want["label"])
+    assert np.array_equal(res["cluster_id"], want["cluster_id"])
