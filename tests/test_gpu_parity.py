@@ -1,10 +1,9 @@
 """Parity of the HIP counting paths with the oracle, through the C ABI.
 
 Bit-exact comparison of every slot count and the triangle total on seeded
-graphs the oracle finishes in seconds; at the full R-MAT scale-22 size,
-size-independent properties (reverse-slot symmetry, sum of counts = 6T,
-T against the oracle's triangle count) plus bit-exact checks of a sample of
-sources.
+graphs, including the full-size R-MAT scale-22 and Chung-Lu 2^23 graphs
+(every slot against the oracle's bit-vector counts), plus size-independent
+properties (reverse-slot symmetry, sum of counts = 6T).
 """
 import ctypes as C
 import hashlib
@@ -292,27 +291,33 @@ def test_paths_agree_rmat20(ctx):
         assert res[p][1] == t, p
 
 
-def test_full_size_rmat22_properties(ctx):
-    """R-MAT scale 22 / ef 16: symmetry, sum = 6T, T equal to the oracle's, and
-    bit-exact counts on 200k uniformly sampled u<v slots and their reverse slots."""
+@pytest.fixture(scope="module")
+def rmat22():
+    """R-MAT scale 22 / ef 16 (the benchmark's graph) with the oracle's
+    bit-vector counts of every slot and its merge-based triangle total."""
     g = G.gen_rmat(22, 16, seed=1)
-    print(f"[rmat22] n={g.n} m={g.m}", flush=True)
+    ref = oracle.cn_bitmap(g)
+    t = oracle.triangle_count(g)
+    print(f"[rmat22] n={g.n} m={g.m} T={t}", flush=True)
+    return g, ref, t
+
+
+@pytest.mark.parametrize("path,params", [("forward", {}), ("forward", {"chunk": 128}), ("bitmap", {}),
+                                         ("merge", {})], ids=["forward", "forward_chunk128", "bitmap", "merge"])
+def test_full_size_rmat22(ctx, rmat22, path, params):
+    """R-MAT scale 22 / ef 16, every one of the 128.3M slots bit-exact against
+    the oracle's counts on each path, T equal to the oracle's triangle count,
+    plus symmetry and sum = 6T."""
+    g, ref, t_ref = rmat22
+    print(f"[rmat22] path={path} params={params}", flush=True)
     with ctx.upload(g) as rg:
-        cnt, t = rg.count_all_edges("forward")
-    rev = g.reverse_slots()
-    assert np.array_equal(cnt, cnt[rev])
+        for k, v in params.items():
+            rg.set_param(k, v)
+        cnt, t = rg.count_all_edges(path)
+    bad = np.nonzero(cnt != ref)[0]
+    assert bad.size == 0, f"{bad.size} slots differ, first {bad[:5]} gpu {cnt[bad[:5]]} ref {ref[bad[:5]]}"
+    assert t == t_ref
     assert int((cnt.astype(np.int64) - 2).sum()) == 6 * t
-    assert t == oracle.triangle_count(g)
-    rng = np.random.default_rng(0)
-    src = g.sources()
-    up = np.nonzero(src < g.edge_dst)[0]
-    sel = rng.choice(up, 200_000, replace=False)
-    ref = np.full(g.m, -1, np.int32)
-    oracle.cn_merge_slots(g, ref, sel, src[sel])
-    # slots written by the sample: the sampled u<v slots and their reverse slots
-    mask = ref >= 0
-    assert mask.sum() > 0
-    assert np.array_equal(cnt[mask], ref[mask])
 
 
 @pytest.mark.parametrize("path", PATHS + ["auto"])
@@ -342,17 +347,17 @@ def test_triangle_total_leaves_counts(ctx):
     assert np.array_equal(cnt, cnt2) and t2 == t
 
 
-def test_triangle_total_rmat22(ctx):
-    g = G.gen_rmat(22, 16, seed=1)
+def test_triangle_total_rmat22(ctx, rmat22):
+    g, _, t_ref = rmat22
     with ctx.upload(g) as rg:
-        assert rg.triangle_total("forward") == oracle.triangle_count(g)
+        assert rg.triangle_total("forward") == t_ref
 
 
 def test_full_size_chunglu23(ctx):
     """Chung-Lu, exponent 2.1, n = 2^23, average degree 32, expected degrees
-    capped at 2^20: symmetry, sum = 6T, T equal to the oracle's, bit-exact
-    counts on 100k sampled u<v slots and their reverse slots, and SCAN at
-    eps 0.3 and 0.6, mu 5, equal to the oracle's SCAN over the same counts."""
+    capped at 2^20: every slot bit-exact against the oracle's counts, T equal
+    to the oracle's, sum = 6T, and SCAN at eps 0.3 and 0.6, mu 5, equal to
+    the oracle's SCAN over the oracle's counts."""
     g = G.gen_chung_lu(1 << 23, 2.1, 32, 1 << 20, seed=1)
     print(f"[chunglu23] n={g.n} m={g.m} max_deg={int(g.degrees().max())}", flush=True)
     with ctx.upload(g) as rg:
@@ -364,21 +369,13 @@ def test_full_size_chunglu23(ctx):
                 ws.run(eps, 5)
                 ws.synchronize()
                 scans[eps] = ws.results()
-    rev = g.reverse_slots()
-    assert np.array_equal(cnt, cnt[rev])
+    ref = oracle.cn_bitmap(g)
+    bad = np.nonzero(cnt != ref)[0]
+    assert bad.size == 0, f"{bad.size} slots differ, first {bad[:5]} gpu {cnt[bad[:5]]} ref {ref[bad[:5]]}"
     assert int((cnt.astype(np.int64) - 2).sum()) == 6 * t
     assert t == oracle.triangle_count(g)
-    rng = np.random.default_rng(1)
-    src = g.sources()
-    up = np.nonzero(src < g.edge_dst)[0]
-    sel = rng.choice(up, 100_000, replace=False)
-    ref = np.full(g.m, -1, np.int32)
-    oracle.cn_merge_slots(g, ref, sel, src[sel])
-    mask = ref >= 0
-    assert mask.sum() == 2 * sel.size
-    assert np.array_equal(cnt[mask], ref[mask])
     for eps, res in scans.items():
-        want = oracle.scan(g, cnt, eps, 5)
+        want = oracle.scan(g, ref, eps, 5)
         assert np.array_equal(res["label"], want["label"]), eps
         core = want["label"] == 1
         assert np.array_equal(res["cluster_id"][core], want["cluster_id"][core]), eps
