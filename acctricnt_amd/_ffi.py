@@ -69,6 +69,8 @@ def graph_lib():
     _sig(lib, "acc_hgraph_reverse_slots", C.c_int, vp, u32p)
     _sig(lib, "acc_write_scan_result", C.c_int, C.c_char_p, C.c_uint32, i32p, i32p, i32p, i32p, C.c_uint64)
     _sig(lib, "acc_scan_result_path", C.c_int, C.c_char_p, C.c_double, C.c_int, C.c_char_p, C.c_int)
+    _sig(lib, "acc_csr_verify_counts", C.c_int, C.c_uint32, C.c_uint64, u32p, i32p, i32p, C.c_int32, C.c_int,
+         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64))
     _graph_lib = lib
     return lib
 

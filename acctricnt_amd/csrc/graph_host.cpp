@@ -525,4 +525,38 @@ int acc_hgraph_reverse_slots(const acc_hgraph *g, uint32_t *rev) {
   return ACC_OK;
 }
 
+int acc_csr_verify_counts(uint32_t n, uint64_t m, const uint32_t *off, const int32_t *adj, const int32_t *cnt,
+                          int32_t base, int nthreads, uint64_t *mismatches, uint64_t *first_bad) {
+  if (!off || (m && (!adj || !cnt)) || !mismatches || !first_bad) return fail(ACC_EINVAL, "bad argument");
+  if (off[0] != 0 || off[n] != m) return fail(ACC_EINVAL, "node_off must run from 0 to m");
+  const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+  const uint64_t words = (static_cast<uint64_t>(n) + 63) / 64;
+  uint64_t bad = 0, first = m;
+#pragma omp parallel num_threads(nt) reduction(+ : bad) reduction(min : first)
+  {
+    // this thread's bit vector of N(u): set for u, probe every N(v) with v > u, clear again
+    std::vector<uint64_t> bits(words ? words : 1, 0);
+#pragma omp for schedule(dynamic, 64)
+    for (long long u = 0; u < static_cast<long long>(n); u++) {
+      const uint64_t s = off[u], e = off[u + 1];
+      for (uint64_t j = s; j < e; j++) bits[static_cast<uint32_t>(adj[j]) >> 6] |= 1ULL << (adj[j] & 63);
+      for (uint64_t j = s; j < e; j++) {
+        const int32_t v = adj[j];
+        if (v <= u || static_cast<uint32_t>(v) >= n) continue;
+        int32_t c = base;
+        for (uint64_t k = off[v]; k < off[v + 1]; k++)
+          c += static_cast<int32_t>((bits[static_cast<uint32_t>(adj[k]) >> 6] >> (adj[k] & 63)) & 1);
+        if (cnt[j] != c) {
+          bad++;
+          first = std::min<uint64_t>(first, j);
+        }
+      }
+      for (uint64_t j = s; j < e; j++) bits[static_cast<uint32_t>(adj[j]) >> 6] = 0;
+    }
+  }
+  *mismatches = bad;
+  *first_bad = first;
+  return ACC_OK;
+}
+
 }  // extern "C"

@@ -250,13 +250,45 @@ void SCAN_XP::CheckCore(Graph *gp) {
   Reorder();
   Upload();
   auto start = high_resolution_clock::now();
-  check_cn(acc_cn_compute(dg_, ACC_PATH_AUTO, 0), "all-edge count");
+  {
+    // per-GPU source ranges of equal work (the reference's
+    // ComputeVertexRangeAccumulatedFilteredDegree, scan_xp_multi_gpu_multi_pass_common.cu:516-554);
+    // this build runs one GPU, whose range is the whole graph
+    auto t0 = high_resolution_clock::now();
+    uint32_t range[2] = {0, 0};
+    check_cn(acc_graph_set_shard(dg_, 1, 0, range), "vertex range");
+    log_info("[%s: %s)", FormatWithCommas(range[0]).c_str(), FormatWithCommas(range[1]).c_str());
+    log_info("task range init by filtered accumulated degree cost : %.3lf s, Mem Usage: %s KB", seconds_since(t0),
+             FormatWithCommas(getValue()).c_str());
+  }
+  check_cn(acc_cn_compute(dg_, ACC_PATH_AUTO, ACC_CN_TIMING), "all-edge count");
   check_cn(acc_ctx_synchronize(ctx_), "all-edge count");
+  // device time of the counting pass from HIP events, kernel by kernel
+  float ms[16] = {0};
+  char names[512] = {0};
+  const int nk = acc_cn_timings(dg_, ms, 16, names, sizeof(names));
+  double kernel_ms = 0, combine_ms = 0;
+  {
+    std::string list(names), name;
+    size_t pos = 0;
+    for (int k = 0; k < nk && k < 16; k++) {
+      const size_t sep = list.find(';', pos);
+      name = list.substr(pos, sep == std::string::npos ? std::string::npos : sep - pos);
+      pos = sep == std::string::npos ? list.size() : sep + 1;
+      kernel_ms += ms[k];
+      if (name == "k_combine") combine_ms = ms[k];
+    }
+  }
+  log_info("CUDA Kernel Time: %.3lf ms", kernel_ms);
+  log_info("CUDA kernel lauch cost in GPU %d: %.3lf s", 0, seconds_since(start));
   log_info("All-Edge Computation Cost (GPU): %.9lf s, %s KB", seconds_since(start),
            FormatWithCommas(getValue()).c_str());
   check_cn(acc_cn_download(dg_, gp->common_node_num, nullptr), "count download");
   log_debug("All-Edge CN-Cnt Cost: %.3lf s, Mem Usage: %s KB", seconds_since(start),
             FormatWithCommas(getValue()).c_str());
+  // the reverse assignment of PostComputeCoreChecking (scan_xp_common.cu:222-236) is the
+  // combine kernel of the pass above, on the device
+  log_info("bin-search cost : %.3lf s, Mem Usage: %s KB", combine_ms / 1000.0, FormatWithCommas(getValue()).c_str());
 
   // similarity + core check (CheckCoreCompSimCore)
   auto s2 = high_resolution_clock::now();
@@ -308,11 +340,49 @@ void SCAN_XP::PrepareResultOutput() {
            FormatWithCommas(getValue()).c_str());
 }
 
+void SCAN_XP::Verify() {
+  double s1 = omp_get_wtime();
+  log_info("Verify with BitVec");
+  uint64_t bad = 0, first = 0;
+  if (acc_csr_verify_counts(g.nodemax, g.edgemax, g.node_off, g.edge_dst, g.common_node_num, 2, thread_num_, &bad,
+                            &first) != ACC_OK)
+    die("verify", acc_graph_last_error());
+  if (bad) {
+    const uint32_t u = static_cast<uint32_t>(std::upper_bound(g.node_off, g.node_off + g.nodemax + 1,
+                                                              static_cast<uint32_t>(first)) - g.node_off) - 1;
+    const int v = g.edge_dst[first];
+    int exp = 2;
+    for (uint32_t a = g.node_off[u], b = g.node_off[v]; a < g.node_off[u + 1] && b < g.node_off[v + 1];) {
+      if (g.edge_dst[a] < g.edge_dst[b]) {
+        a++;
+      } else if (g.edge_dst[a] > g.edge_dst[b]) {
+        b++;
+      } else {
+        exp++;
+        a++;
+        b++;
+      }
+    }
+    log_info("Verify Fail, Fix Bug Now!");
+    log_info("%u, %d, cur: %d, exp: %d", u, v, g.common_node_num[first], exp);
+    log_info("%s u<v slots differ", FormatWithCommas(static_cast<long long>(bad)).c_str());
+    std::exit(-1);
+  }
+  double e1 = omp_get_wtime();
+  log_info("Verification Time Cost: %.3lf s, Mem Usage: %s KB", e1 - s1, FormatWithCommas(getValue()).c_str());
+  log_info("Verify Pass, Correct!");
+}
+
 void SCAN_XP::Execute() {
   double s1 = omp_get_wtime();
   CheckCore(&g);
   double e1 = omp_get_wtime();
   log_info("Step1 - CheckCore: %.3lf s, Mem Usage: %s KB", e1 - s1, FormatWithCommas(getValue()).c_str());
+
+  // the reference always verifies after CheckCore (scan_xp_common.cu:337-340); here it is
+  // opt-in, as a CPU recount costs far more than the GPU pass it checks
+  const char *verify = std::getenv("SCANXP_VERIFY");
+  if (verify && verify[0] && std::strcmp(verify, "0") != 0) Verify();
 
   double s2 = omp_get_wtime();
   ClusterCore();

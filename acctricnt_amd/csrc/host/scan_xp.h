@@ -94,6 +94,9 @@ class SCAN_XP {
   void LabelNonCore();
   void PostProcess();
   void PrepareResultOutput();
+  // CPU recount of the u<v slots' counts with a bit vector (the reference's
+  // Verify, scan-query-cuda/scan_xp_common.cu:270-323); exits with -1 on a mismatch.
+  void Verify();
 };
 
 // Parses a decimal spelling ("0.2", "1", ".35", "2e-1") into an exact fraction.

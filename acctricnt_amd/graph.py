@@ -57,6 +57,33 @@ class CSRGraph:
         smin = np.minimum(d[src[up]], d[self.edge_dst[up]]).sum(dtype=np.int64)
         return int(4 * (self.n + 1) + 8 * self.m + 4 * int(smin))
 
+    def forward_bytes(self, triangles: int, rev: np.ndarray | None = None) -> dict:
+        """Algorithmic bytes of one forward counting launch (every triangle
+        {a < b < c} found once at its middle vertex b by probing N(c) ∩ [0, b)
+        against N<(b)), term by term:
+          offsets      4 (n + 1)               read once;
+          lower_lists  4 * m/2                 N<(b) of every source, read once;
+          upper_slots  16 * m/2                per upper slot (b, c): c, its reverse
+                                               slot and the start of N(c) read, the
+                                               slot's count written;
+          probes       4 * P                   P = sum over upper slots (b, c) of
+                                               |N(c) ∩ [0, b)| = rev(b, c) - start of N(c);
+          updates      4 * T + 4 * m/2         one 4-byte update of the lower slot
+                                               (c, a) per triangle, one flush per
+                                               lower slot (b, a).
+        These are the bytes the kernel cannot avoid moving; HBM traffic above
+        them is re-reading, below them is cache reuse."""
+        if rev is None:
+            rev = self.reverse_slots()
+        src = self.sources()
+        up = src < self.edge_dst
+        off = self.node_off.astype(np.int64)
+        probes = int((rev[up].astype(np.int64) - off[self.edge_dst[up]]).sum())
+        half = self.m // 2
+        terms = {"offsets": 4 * (self.n + 1), "lower_lists": 4 * half, "upper_slots": 16 * half,
+                 "probes": 4 * probes, "updates": 4 * int(triangles) + 4 * half}
+        return {"bytes": int(sum(terms.values())), "terms": terms, "probe_elements": probes}
+
 
 def _from_handle(h, name="", meta=None) -> CSRGraph:
     lib = graph_lib()
@@ -123,6 +150,19 @@ def from_csr(node_off, edge_dst, name: str = "csr") -> CSRGraph:
 def validate(g: CSRGraph) -> None:
     check_graph(graph_lib().acc_csr_validate(g.n, g.m, _ffi.ptr(g.node_off, C.c_uint32),
                                              _ffi.ptr(g.edge_dst, C.c_int32)))
+
+
+def verify_counts(g: CSRGraph, cnt, base: int = 2, nthreads: int = 0) -> tuple[int, int]:
+    """CPU bit-vector recount of the u<v slots of a count array (the drop-in's
+    Verify step): (number of mismatching u<v slots, first such slot or m)."""
+    cnt = np.ascontiguousarray(cnt, dtype=np.int32)
+    if cnt.shape[0] != g.m:
+        raise ValueError("count array length differs from m")
+    bad, first = C.c_uint64(), C.c_uint64()
+    check_graph(graph_lib().acc_csr_verify_counts(g.n, g.m, _ffi.ptr(g.node_off, C.c_uint32),
+                                                  _ffi.ptr(g.edge_dst, C.c_int32), _ffi.ptr(cnt, C.c_int32),
+                                                  int(base), int(nthreads), C.byref(bad), C.byref(first)))
+    return bad.value, first.value
 
 
 def gen_er(n: int, avg_deg: float, seed: int, relabel: bool = False) -> CSRGraph:

@@ -111,6 +111,17 @@ int acc_scan_result_path(const char *dir, double eps, int mu, char *buf, int cap
 /* Reverse-slot index on the host: rev[i] = slot of (v,u) for slot i = (u,v). */
 int acc_hgraph_reverse_slots(const acc_hgraph *g, uint32_t *rev_out);
 
+/* CPU recount of a count array, the reference's Verify step after CheckCore
+ * (scan-query-cuda/scan_xp_common.cu:270-323, BIT_VEC form): for every slot
+ * i = (u, v) with u < v, cnt[i] must equal base + |N(u) ∩ N(v)| (base 2 for
+ * the reference's layout, 0 for raw counts), recounted with a per-thread bit
+ * vector of N(u) probed by N(v).  *mismatches receives the number of such
+ * slots that differ and *first_bad the smallest of them (m when none).  The
+ * CSR is trusted (validated at upload).  nthreads <= 0: all threads. */
+int acc_csr_verify_counts(uint32_t n, uint64_t m, const uint32_t *node_off, const int32_t *edge_dst,
+                          const int32_t *cnt, int32_t base, int nthreads, uint64_t *mismatches,
+                          uint64_t *first_bad);
+
 #ifdef __cplusplus
 }
 #endif

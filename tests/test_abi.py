@@ -164,3 +164,24 @@ def test_result_path_format(tmp_path):
 
     assert cn.scan_result_path("/d", 0.2, 5) == "/d/scanxp-result-0.2-5.txt"
     assert cn.scan_result_path("/d", 1.0, 2) == "/d/scanxp-result-1.-2.txt"
+
+
+@pytest.mark.parametrize("maker", [lambda: G.gen_rmat(11, 16, seed=21), lambda: BF.wheel(9),
+                                   lambda: G.gen_chung_lu(3000, 2.1, 12, 600, seed=22)])
+def test_verify_counts_recount(maker):
+    """The drop-in's Verify recount (acc_csr_verify_counts) accepts scipy's
+    per-slot counts, and reports exactly the u<v slots where a count was
+    changed (a change in a lower slot is outside the check, as in the
+    reference's Verify, which recounts u<v slots only)."""
+    g = maker()
+    ref = BF.cn_scipy(g).astype(np.int32)
+    assert G.verify_counts(g, ref) == (0, g.m)
+    assert G.verify_counts(g, ref - 2, base=0) == (0, g.m)
+    src = g.sources()
+    up = np.nonzero(src < g.edge_dst)[0]
+    low = np.nonzero(src > g.edge_dst)[0]
+    bad = ref.copy()
+    pick = np.sort(np.random.default_rng(3).choice(up, min(5, up.size), replace=False))
+    bad[pick] += 1
+    bad[low[0]] -= 1
+    assert G.verify_counts(g, bad, nthreads=3) == (pick.size, int(pick[0]))

@@ -16,9 +16,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "acctricnt_amd", "bin")
 
 
-def run(exe, *args, timeout=120):
+def run(exe, *args, timeout=120, env=None):
     return subprocess.run([os.path.join(BIN, exe), *map(str, args)], capture_output=True, text=True,
-                          timeout=timeout)
+                          timeout=timeout, env=None if env is None else {**os.environ, **env})
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -49,6 +49,26 @@ def test_self_loop_exits_1(tmp_path):
     assert "Self loop" in r.stdout
 
 
+def harness_intersection_time(text):
+    """The reference harness's "O(E) intersection count time"
+    (python_experiments/data_analysis/common/varying_datasets_cuda_exec.py:110-116):
+    Step1 minus the similarity cost minus the task-range init, each parsed as
+    get_time_mem_info_updated does (read_file_utils_updated.py:13-15, 47-71):
+    the message after the log header's 10 fields, the time before 's,'."""
+    header = len("2019-03-23 11:05:57 INFO  (ts: 1553310357.245065 s,  et: 0.000756 s) main_scanxp.cpp:40:".split())
+
+    def tag_time(tag):
+        for line in text.splitlines():
+            rest = " ".join(line.split()[header:])
+            if rest.startswith(tag):
+                return float(rest.split("s,")[0].split(":")[-1].split()[0])
+        return None
+
+    t = [tag_time(x) for x in ("Step1 - CheckCore:", "core-checking sim-core-comp cost",
+                               "task range init by filtered accumulated degree cost")]
+    return None if None in t else t[0] - t[1] - t[2]
+
+
 def _count(text, tag):
     m = re.search(r"\b" + tag + r": (\d[\d,.]*)", text)
     return int(re.sub(r"[^0-9]", "", m.group(1))) if m else None
@@ -65,8 +85,14 @@ def test_k4_pendant_reference_run(tmp_path):
     text = log.read_text()
     assert "Core: 4" in text and re.search(r"Cluster: 1, Hub: 0, Outlier: 1", text)
     for tag in ("Step1 - CheckCore:", "Step2 - ClusterCore", "Step3 - LabelNonCore", "Step4 - output to the disk cost",
-                "All-Edge Computation"):
+                "All-Edge Computation", "CUDA Kernel Time", "CUDA kernel lauch cost in GPU 0", "bin-search cost",
+                "task range init by filtered accumulated degree cost", "core-checking sim-core-comp cost"):
         assert tag in text, tag
+    assert "Verify" not in text  # opt-in
+    kernel = re.search(r"CUDA Kernel Time: ([0-9.]+) ms", text)
+    assert kernel and float(kernel.group(1)) > 0
+    t = harness_intersection_time(text)
+    assert t is not None and t >= 0
     res = (tmp_path / "scanxp-result-0.6-2.txt").read_text().splitlines()
     assert res == ["c/n vertex_id cluster_id", "c 0 0", "c 1 0", "c 2 0", "c 3 0", "n 4 0"]
     r = run("main_triangle_count", tmp_path, "4", "deg", log)
@@ -119,7 +145,7 @@ def test_scan_result_file_full_size_rmat22(tmp_path):
     G.write_bins(g, str(tmp_path))
     eps, mu = "0.2", 5
     log = tmp_path / "log.txt"
-    r = run("main_scanxp", tmp_path, eps, mu, 16, "deg", log, timeout=280)
+    r = run("main_scanxp", tmp_path, eps, mu, 16, "deg", log, timeout=280, env={"SCANXP_VERIFY": "1"})
     assert r.returncode == 0, r.stderr
     ref = oracle.scan(g, oracle.cn_bitmap(g), eps, mu)
     expect = tmp_path / "expect.txt"
@@ -128,3 +154,6 @@ def test_scan_result_file_full_size_rmat22(tmp_path):
     assert out.read_bytes() == expect.read_bytes()
     text = log.read_text()
     assert _count(text, "Core") == ref["core_num"]
+    assert "Verify with BitVec" in text and "Verify Pass, Correct!" in text
+    assert "Verify Fail" not in text
+    print(f"[drop-in rmat22] harness intersection time {harness_intersection_time(text):.3f} s", flush=True)
