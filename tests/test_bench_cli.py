@@ -7,6 +7,8 @@ import sys
 
 import pytest
 
+from acctricnt_amd import graph as G
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -57,3 +59,27 @@ def test_every_profile_names_its_command(name):
     assert cmds, name
     for c in cmds:
         assert " -- python3 bench.py" in c and "_work" not in c, c
+
+
+@pytest.mark.parametrize("maker", [lambda: G.gen_rmat(9, 8, seed=31), lambda: G.gen_er(300, 6, seed=32),
+                                   lambda: G.gen_chung_lu(400, 2.1, 8, 100, seed=33)])
+def test_forward_bytes_by_brute_force(maker):
+    """The forward kernel's algorithmic bytes: the probe count P is the sum over
+    upper slots (b, c) of |{x in N(c) : x < b}|, counted here from the lists;
+    the other terms follow from n, m and T."""
+    import oracle
+
+    g = maker()
+    t = oracle.triangle_count(g)
+    off, adj = g.node_off, g.edge_dst
+    p = 0
+    for b in range(g.n):
+        for c in adj[off[b]:off[b + 1]]:
+            if c > b:
+                p += int((adj[off[c]:off[c + 1]] < b).sum())
+    fb = g.forward_bytes(t)
+    assert fb["probe_elements"] == p
+    half = g.m // 2
+    assert fb["terms"] == {"offsets": 4 * (g.n + 1), "lower_lists": 4 * half, "upper_slots": 16 * half,
+                           "probes": 4 * p, "updates": 4 * t + 4 * half}
+    assert fb["bytes"] == sum(fb["terms"].values())
