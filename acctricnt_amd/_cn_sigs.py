@@ -44,6 +44,7 @@ SIGNATURES = {
     "acc_cn_timings": (C.c_int, [vp, f32p, C.c_int, C.c_char_p, C.c_int]),
     "acc_triangle_total": (C.c_int, [vp, C.c_int, u64p]),
     "acc_triangle_launch": (C.c_int, [vp]),
+    "acc_triangle_total_device": (C.c_int, [vp, C.c_uint32, C.c_uint64, vp, vp, vp]),
     "acc_graph_set_param": (C.c_int, [vp, C.c_int, C.c_int64]),
     "acc_graph_get_param": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64)]),
     "acc_graph_set_partition": (C.c_int, [vp, C.c_uint32, u32p]),

@@ -45,7 +45,14 @@ struct acc_ctx {
   size_t lds_per_block = 0;
   std::mutex copy_mu;             // the copy lanes serve one transfer at a time
   std::vector<CopyLane> lanes;    // allocated at the first large pageable transfer
+  unsigned long long *na_scratch = nullptr;  // device address of the allocation-free entry point's scratch
 };
+
+namespace accnoalloc {
+// Device address of acc_noalloc.hip's file-scope scratch array on `device`
+// (the current device), looked up once per device.
+int bind_scratch(int device, unsigned long long **addr);
+}  // namespace accnoalloc
 
 namespace accfwd {
 constexpr int kWG = 512;                              // threads per workgroup (8 waves)

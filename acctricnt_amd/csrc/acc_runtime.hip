@@ -471,6 +471,12 @@ int acc_ctx_create(int device, acc_ctx **out) {
     delete ctx;
     return acc_set_error(ACC_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
+  const int rc = accnoalloc::bind_scratch(device, &ctx->na_scratch);
+  if (rc) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return ACC_OK;
 }

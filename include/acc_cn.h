@@ -172,6 +172,23 @@ int acc_triangle_total(acc_graph *g, int path, uint64_t *triangles);
  * (for timing; the total is what acc_triangle_total returns). */
 int acc_triangle_launch(acc_graph *g);
 
+/* Allocation-free triangle total.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops and harnesses that forbid hipMalloc).  Enqueues on the
+ * context stream the triangle count T of a CSR that already lies in device
+ * memory (uint32 d_node_off[n+1], int32 d_edge_dst[m]; sorted lists without
+ * self-loops, symmetric) and writes T into *d_triangles (device memory) on
+ * completion; it allocates nothing and does not wait.  Its only other device
+ * memory is a fixed 16.6 KB array inside the library (one partial sum per
+ * workgroup of a grid of at most 2048 workgroups, and an error word), one per
+ * device, whatever the graph's size; calls that overlap in time on one device
+ * share it and race.  An offset outside [0, m] or an id outside [0, n) is
+ * detected without reading outside the arrays and makes *d_triangles ~0.
+ * ACC_EINVAL for n >= 2^31 or m > 2^32 - 256.  The same T as
+ * acc_triangle_total (SCAN_XP::TriCnt, scan-query/scan_xp_triangle_cnt.cpp:778-957). */
+int acc_triangle_total_device(acc_ctx *ctx, uint32_t n, uint64_t m, const uint32_t *d_node_off,
+                              const int32_t *d_edge_dst, unsigned long long *d_triangles);
+
 /* SCAN queries ---------------------------------------------------------- */
 /* Structural clustering on the resident counts, replacing the reference's
  * post-processing after CheckCore: PostComputeCoreChecking / CheckCoreCompSimCore
