@@ -26,7 +26,8 @@ PATH_BITMAP = 2
 PATH_MERGE = 4
 PATHS = {"auto": PATH_AUTO, "forward": PATH_FORWARD, "bitmap": PATH_BITMAP, "merge": PATH_MERGE}
 
-PARAMS = {"fwd_unroll": 1, "wave_slots": 2, "wave_low": 3, "chunk": 4, "auto_path": 5, "persist": 6}
+PARAMS = {"fwd_unroll": 1, "wave_slots": 2, "wave_low": 3, "chunk": 4, "auto_path": 5, "persist": 6,
+          "bmp_hash_max": 7, "bmp_batch": 8}
 TUNE_QUICK = 0x1
 TUNE_FORWARD_ONLY = 0x2
 

@@ -76,6 +76,31 @@ struct FwdUnit {
   uint32_t nlow;
 };
 
+// Per-edge plans (bitmap and merge paths): chunks of at most kChunk upper
+// slots of every source, by d_u.  Wave units: a source with d_u <= 256 and at
+// most 64 upper slots, one wave each, N(u) in a wave-private LDS table or
+// sorted run.  Block units: the bitmap path keeps N(u) in an LDS hash table
+// of 2^12, 2^14 or 2^15 keys (d_u up to half of that) or, for hubs, in an HBM
+// bitmap; the merge path stages N(u) in LDS windows of 2^11 ids (the first
+// class) or 2^13 ids (the others).
+enum EdgeClass { kEdgeWave = 0, kEdgeHash12, kEdgeHash14, kEdgeHash15, kEdgeHub, kEdgeClasses };
+constexpr uint32_t kEdgeWaveDeg = 256;  // d_u of a wave unit
+constexpr uint32_t kEdgeWaveUp = 64;    // upper slots of a wave unit
+struct EdgePlan {
+  FwdUnit *d_units[kEdgeClasses] = {};  // hub units: nlow = the hub's bitmap in its batch's pool
+  uint32_t n_units[kEdgeClasses] = {};
+  uint32_t *d_hubs = nullptr;           // hub sources in order
+  uint32_t n_hubs = 0;
+  std::vector<uint32_t> hub_batch;       // batch k: hubs [hub_batch[k], hub_batch[k+1])
+  std::vector<uint32_t> hub_unit_batch;  // ... and hub units [hub_unit_batch[k], hub_unit_batch[k+1])
+  uint32_t *d_pool = nullptr;            // bitmaps of one batch of hubs, zero between passes
+  uint32_t pool_hubs = 0;
+  uint64_t pool_words = 0;               // words per bitmap: ceil(n / 32)
+  int idx_shift = 0;                     // LDS index: one bit per 2^idx_shift bitmap words
+  bool valid = false;
+};
+constexpr uint32_t kEdgeHashMax = 1u << 14;  // largest d_u an LDS hash table holds
+
 struct acc_graph {
   acc_ctx *ctx = nullptr;
   uint32_t n = 0;
@@ -107,17 +132,7 @@ struct acc_graph {
   uint32_t n_fwd_block = 0;
   FwdUnit *d_fwd_big = nullptr;  // |N<(b)| beyond the LDS table: global-memory lookups
   uint32_t n_fwd_big = 0;
-  // per-edge plans (bitmap and merge paths): chunks of upper slots of every source
-  FwdUnit *d_edge_units = nullptr;  // all sources (merge path)
-  uint32_t n_edge_units = 0;
-  FwdUnit *d_bmp_lds = nullptr;  // sources with d_u <= kLdsCap (bitmap path, LDS set)
-  uint32_t n_bmp_lds = 0;
-  FwdUnit *d_bmp_hub = nullptr;  // hub sources (bitmap path, HBM bitmap)
-  uint32_t n_bmp_hub = 0;
-  bool edge_plans_valid = false;  // per-edge plans are built at the first bitmap or merge pass
-  uint32_t *d_bmp_pool = nullptr;  // one zeroed n-bit bitmap per persistent workgroup
-  uint32_t bmp_pool_slots = 0;
-  uint64_t bmp_pool_words = 0;
+  EdgePlan edge;  // per-edge plans, built at the first bitmap or merge pass
   uint64_t device_bytes = 0;
   // tuning parameters (acc_graph_set_param)
   int fwd_unroll = 4;
@@ -126,6 +141,8 @@ struct acc_graph {
   uint32_t wave_low = accfwd::kWaveCap;      // ... and |N<(b)| at most this
   uint32_t chunk = accfwd::kChunk;           // upper slots per workgroup unit
   uint32_t persist = 0;                      // counting grid: 0 one workgroup per task, else per-CU workgroups
+  uint32_t bmp_hash_max = 256;               // bitmap path: d_u above this goes to the HBM bitmaps
+  uint32_t bmp_batch = 4096;                 // bitmap path: hub bitmaps built and probed per batch
   uint32_t shard_lo = 0, shard_hi = 0;       // forward sources [shard_lo, shard_hi) (the whole graph: 0, n)
   std::vector<uint64_t> h_work;              // prefix of the forward probe work per source (n + 1), for shards
   // host copies kept for plan rebuilds: offsets, first upper slot and smallest

@@ -35,9 +35,11 @@ void free_graph(acc_graph *g) {
                   static_cast<void *>(g->d_rev), static_cast<void *>(g->d_sum), static_cast<void *>(g->d_flags),
                   static_cast<void *>(g->d_fwd_wave), static_cast<void *>(g->d_fwd_block),
                   static_cast<void *>(g->d_fwd_big), static_cast<void *>(g->d_src),
-                  static_cast<void *>(g->d_old), static_cast<void *>(g->d_edge_units),
-                  static_cast<void *>(g->d_bmp_lds), static_cast<void *>(g->d_bmp_hub),
-                  static_cast<void *>(g->d_bmp_pool), static_cast<void *>(g->d_skey),
+                  static_cast<void *>(g->d_old), static_cast<void *>(g->edge.d_units[kEdgeWave]),
+                  static_cast<void *>(g->edge.d_units[kEdgeHash12]),
+                  static_cast<void *>(g->edge.d_units[kEdgeHash14]), static_cast<void *>(g->edge.d_units[kEdgeHash15]),
+                  static_cast<void *>(g->edge.d_units[kEdgeHub]), static_cast<void *>(g->edge.d_hubs),
+                  static_cast<void *>(g->edge.d_pool), static_cast<void *>(g->d_skey),
                   static_cast<void *>(g->d_snbr), static_cast<void *>(g->d_sslot),
                   static_cast<void *>(g->d_tri_parts), static_cast<void *>(g->d_hs)})
     if (p) (void)hipFree(p);
@@ -161,18 +163,42 @@ void build_forward_plan(uint32_t b_lo, uint32_t b_hi, const uint32_t *off, const
   }
 }
 
-// Per-edge plan: chunks of at most `chunk` upper slots of every source (merge
-// path); the bitmap path splits them by d_u into LDS-set and hub units.
-void build_edge_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, uint32_t chunk, std::vector<FwdUnit> &all,
-                     std::vector<FwdUnit> &lds, std::vector<FwdUnit> &hub) {
+// Per-edge plan: chunks of at most `chunk` upper slots of every source, by
+// d_u: wave units (d_u <= 256 and at most 64 upper slots, one unit per
+// source), LDS hash classes of 2^12, 2^14 and 2^15 keys for d_u up to 2048,
+// 8192 and min(16384, hash_max); the rest are hubs.  Hubs go into batches of at
+// most pool_hubs (one HBM bitmap each); a hub unit's nlow is its hub's
+// bitmap within the batch.
+constexpr uint64_t kHubPoolBytes = 8ull << 30;  // HBM bitmaps of one hub batch
+constexpr uint32_t kHubBatch = 16384;           // most hubs of one batch (ACC_PARAM_BMP_BATCH)
+constexpr uint32_t kHubIdxBits = 1u << 17;      // LDS index bits of a hub unit (k_bmp_hub's kIdxWords)
+void build_edge_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, uint32_t chunk, uint32_t hash_max,
+                     uint32_t pool_hubs, std::vector<FwdUnit> (&units)[kEdgeClasses], std::vector<uint32_t> &hubs,
+                     std::vector<uint32_t> &hub_batch, std::vector<uint32_t> &hub_unit_batch) {
+  hub_batch.assign(1, 0);
+  hub_unit_batch.assign(1, 0);
   for (uint32_t u = 0; u < n; u++) {
     const uint32_t du = off[u + 1] - off[u];
-    for (uint32_t j0 = hs[u]; j0 < off[u + 1]; j0 += chunk) {
-      FwdUnit un{static_cast<int32_t>(u), j0, std::min(j0 + chunk, off[u + 1]), du};
-      all.push_back(un);
-      (du <= accfwd::kLdsCap ? lds : hub).push_back(un);
+    if (hs[u] == off[u + 1]) continue;
+    int c = kEdgeHub;
+    if (du <= kEdgeWaveDeg && off[u + 1] - hs[u] <= kEdgeWaveUp)
+      c = kEdgeWave;
+    else if (du <= hash_max)
+      c = du <= 2048 ? kEdgeHash12 : du <= 8192 ? kEdgeHash14 : kEdgeHash15;
+    uint32_t slot = 0;
+    if (c == kEdgeHub) {
+      if (hubs.size() - hub_batch.back() == pool_hubs) {
+        hub_batch.push_back(static_cast<uint32_t>(hubs.size()));
+        hub_unit_batch.push_back(static_cast<uint32_t>(units[kEdgeHub].size()));
+      }
+      slot = static_cast<uint32_t>(hubs.size()) - hub_batch.back();
+      hubs.push_back(u);
     }
+    for (uint32_t j0 = hs[u]; j0 < off[u + 1]; j0 += chunk)
+      units[c].push_back(FwdUnit{static_cast<int32_t>(u), j0, std::min(j0 + chunk, off[u + 1]), c == kEdgeHub ? slot : du});
   }
+  hub_batch.push_back(static_cast<uint32_t>(hubs.size()));
+  hub_unit_batch.push_back(static_cast<uint32_t>(units[kEdgeHub].size()));
 }
 
 template <typename T>
@@ -227,32 +253,53 @@ int upload_plans(acc_graph *g) {
   PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},
                       {&block, &g->d_fwd_block, &g->n_fwd_block},
                       {&big, &g->d_fwd_big, &g->n_fwd_big}};
-  g->edge_plans_valid = false;
+  g->edge.valid = false;
   return upload_lists(g, lists);
 }
 
 // The per-edge (bitmap and merge) plans, built at the first pass that needs
 // them: most graphs only ever run the forward path.
 int ensure_edge_plans(acc_graph *g) {
-  if (g->edge_plans_valid) return ACC_OK;
-  std::vector<FwdUnit> eall, elds, ehub;
-  build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, eall, elds, ehub);
-  PlanList lists[] = {{&eall, &g->d_edge_units, &g->n_edge_units},
-                      {&elds, &g->d_bmp_lds, &g->n_bmp_lds},
-                      {&ehub, &g->d_bmp_hub, &g->n_bmp_hub}};
+  EdgePlan &p = g->edge;
+  if (p.valid) return ACC_OK;
+  p.pool_words = std::max<uint64_t>(1, (static_cast<uint64_t>(g->n) + 31) / 32);
+  p.idx_shift = 0;
+  while (((p.pool_words + (1ull << p.idx_shift) - 1) >> p.idx_shift) > kHubIdxBits) p.idx_shift++;
+  const uint32_t pool_hubs = static_cast<uint32_t>(
+      std::min<uint64_t>(g->bmp_batch, std::max<uint64_t>(1, kHubPoolBytes / (sizeof(uint32_t) * p.pool_words))));
+  std::vector<FwdUnit> units[kEdgeClasses];
+  std::vector<uint32_t> hubs;
+  build_edge_plan(g->n, g->h_off.data(), g->h_hs.data(), g->chunk, g->bmp_hash_max, pool_hubs, units, hubs,
+                  p.hub_batch, p.hub_unit_batch);
+  PlanList lists[] = {{&units[kEdgeWave], &p.d_units[kEdgeWave], &p.n_units[kEdgeWave]},
+                      {&units[kEdgeHash12], &p.d_units[kEdgeHash12], &p.n_units[kEdgeHash12]},
+                      {&units[kEdgeHash14], &p.d_units[kEdgeHash14], &p.n_units[kEdgeHash14]},
+                      {&units[kEdgeHash15], &p.d_units[kEdgeHash15], &p.n_units[kEdgeHash15]},
+                      {&units[kEdgeHub], &p.d_units[kEdgeHub], &p.n_units[kEdgeHub]}};
   ACC_TRY(upload_lists(g, lists));
-  g->edge_plans_valid = true;
+  dev_free(g, &p.d_hubs, p.n_hubs);
+  p.n_hubs = static_cast<uint32_t>(hubs.size());
+  if (p.n_hubs) {
+    ACC_TRY(dev_alloc(g, &p.d_hubs, p.n_hubs));
+    ACC_HIP_TRY(hipMemcpyAsync(p.d_hubs, hubs.data(), sizeof(uint32_t) * p.n_hubs, hipMemcpyHostToDevice,
+                               g->ctx->stream));
+    ACC_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  }
+  p.valid = true;
   return ACC_OK;
 }
 
+// The hub bitmap pool (zeroed once: every bitmap pass clears what it sets).
 int ensure_bitmap_pool(acc_graph *g) {
-  if (g->d_bmp_pool || !g->n_bmp_hub) return ACC_OK;
-  const uint32_t slots = static_cast<uint32_t>(std::max(1, g->ctx->num_cus) * 2);
-  const uint64_t words = (static_cast<uint64_t>(g->n) + 31) / 32 + 1;
-  ACC_TRY(dev_alloc(g, &g->d_bmp_pool, words * slots));
-  ACC_HIP_TRY(hipMemsetAsync(g->d_bmp_pool, 0, sizeof(uint32_t) * words * slots, g->ctx->stream));
-  g->bmp_pool_slots = slots;
-  g->bmp_pool_words = words;
+  EdgePlan &p = g->edge;
+  uint32_t need = 0;
+  for (size_t k = 0; k + 1 < p.hub_batch.size(); k++) need = std::max(need, p.hub_batch[k + 1] - p.hub_batch[k]);
+  if (need <= p.pool_hubs) return ACC_OK;
+  dev_free(g, &p.d_pool, static_cast<uint64_t>(p.pool_hubs) * p.pool_words);
+  p.pool_hubs = 0;
+  ACC_TRY(dev_alloc(g, &p.d_pool, static_cast<uint64_t>(need) * p.pool_words));
+  ACC_HIP_TRY(hipMemsetAsync(p.d_pool, 0, sizeof(uint32_t) * need * p.pool_words, g->ctx->stream));
+  p.pool_hubs = need;
   return ACC_OK;
 }
 
@@ -725,6 +772,18 @@ int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
         return acc_set_error(ACC_EINVAL, "auto_path must be a counting path");
       g->auto_path = static_cast<int>(value);
       return ACC_OK;
+    case ACC_PARAM_BMP_HASH_MAX:
+      if (value < 0 || value > static_cast<int64_t>(kEdgeHashMax))
+        return acc_set_error(ACC_EINVAL, "bmp_hash_max must lie in [0, 16384]");
+      if (static_cast<uint32_t>(value) != g->bmp_hash_max) g->edge.valid = false;
+      g->bmp_hash_max = static_cast<uint32_t>(value);
+      return ACC_OK;
+    case ACC_PARAM_BMP_BATCH:
+      if (value < 1 || value > static_cast<int64_t>(kHubBatch))
+        return acc_set_error(ACC_EINVAL, "bmp_batch must lie in [1, 16384]");
+      if (static_cast<uint32_t>(value) != g->bmp_batch) g->edge.valid = false;
+      g->bmp_batch = static_cast<uint32_t>(value);
+      return ACC_OK;
     default:
       return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
   }
@@ -739,6 +798,8 @@ int acc_graph_get_param(const acc_graph *g, int param, int64_t *value) {
     case ACC_PARAM_CHUNK: *value = g->chunk; return ACC_OK;
     case ACC_PARAM_AUTO_PATH: *value = g->auto_path; return ACC_OK;
     case ACC_PARAM_PERSIST: *value = g->persist; return ACC_OK;
+    case ACC_PARAM_BMP_HASH_MAX: *value = g->bmp_hash_max; return ACC_OK;
+    case ACC_PARAM_BMP_BATCH: *value = g->bmp_batch; return ACC_OK;
     default: return acc_set_error(ACC_EINVAL, "unknown parameter " + std::to_string(param));
   }
 }
@@ -809,14 +870,30 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap) {
   }
   int best_path = ACC_PATH_FORWARD;
   if (!(flags & ACC_TUNE_FORWARD_ONLY)) {
-    for (int path : {ACC_PATH_BITMAP, ACC_PATH_MERGE}) {
+    // the bitmap path's hash/bitmap boundary, then both per-edge paths against the forward path
+    float bmp_ms = 0;
+    int64_t keep = g->bmp_hash_max;
+    for (int64_t v : quick ? std::vector<int64_t>{256} : std::vector<int64_t>{256, 2048, kEdgeHashMax}) {
+      ACC_TRY(acc_graph_set_param(g, ACC_PARAM_BMP_HASH_MAX, v));
       float ms = 0;
-      ACC_TRY(time_pass(g, path, 1, &ms));
-      add_trial(path == ACC_PATH_BITMAP ? "bitmap" : "merge", "path", path, ms);
-      if (ms < best_ms) {
-        best_ms = ms;
-        best_path = path;
+      ACC_TRY(time_pass(g, ACC_PATH_BITMAP, 1, &ms));
+      add_trial("bitmap", "bmp_hash_max", v, ms);
+      if (bmp_ms == 0 || ms < bmp_ms) {
+        bmp_ms = ms;
+        keep = v;
       }
+    }
+    ACC_TRY(acc_graph_set_param(g, ACC_PARAM_BMP_HASH_MAX, keep));
+    if (bmp_ms < best_ms) {
+      best_ms = bmp_ms;
+      best_path = ACC_PATH_BITMAP;
+    }
+    float ms = 0;
+    ACC_TRY(time_pass(g, ACC_PATH_MERGE, 1, &ms));
+    add_trial("merge", "path", ACC_PATH_MERGE, ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best_path = ACC_PATH_MERGE;
     }
   }
   ACC_TRY(acc_graph_set_param(g, ACC_PARAM_AUTO_PATH, best_path));

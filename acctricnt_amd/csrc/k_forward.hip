@@ -308,15 +308,23 @@ __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__
     int32_t a[U];
     uint32_t idx[U];
     int kq[U];
+    // the U searches advance together, one LDS read of each per step, so
+    // their latencies overlap instead of adding up
+#pragma unroll
+    for (int q = 0; q < U; q++) kq[q] = 0;
+#pragma unroll
+    for (int st = 256; st; st >>= 1) {
+      uint32_t p[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) p[q] = s_P[kq[q] + st];
+#pragma unroll
+      for (int q = 0; q < U; q++)
+        if (p[q] <= e0 + 64 * q + lane) kq[q] += st;
+    }
 #pragma unroll
     for (int q = 0; q < U; q++) {
       const uint32_t e = e0 + 64 * q + lane;
-      int k = 0;
-#pragma unroll
-      for (int st = 256; st; st >>= 1)
-        if (s_P[k + st] <= e) k += st;
-      kq[q] = k;
-      idx[q] = s_cb[k] + e;
+      idx[q] = s_cb[kq[q]] + e;
       a[q] = e < T ? adj[idx[q]] : -1;
     }
 #pragma unroll
@@ -436,6 +444,22 @@ __device__ __forceinline__ int lane_search(uint32_t X, uint32_t e) {
   return k;
 }
 
+// U such searches advanced together (one lane exchange of each per step).
+template <int U>
+__device__ __forceinline__ void lane_search_n(uint32_t X, const uint32_t (&e)[U], int (&k)[U]) {
+#pragma unroll
+  for (int q = 0; q < U; q++) k[q] = 0;
+#pragma unroll
+  for (int st = 32; st; st >>= 1) {
+    uint32_t v[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) v[q] = static_cast<uint32_t>(__shfl(X, k[q] + st));
+#pragma unroll
+    for (int q = 0; q < U; q++)
+      if (v[q] <= e[q]) k[q] += st;
+  }
+}
+
 // Hash table size of one source of a wave pack: the smallest power of two
 // >= 2 |N<(b)|, at least 16 entries.
 __device__ __forceinline__ int pack_bits(uint32_t nl) {
@@ -546,13 +570,14 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
     int32_t a[U];
     uint32_t idx[U];
     int kq[U];
+    uint32_t eq[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) eq[q] = e0 + 64 * q + lane;
+    lane_search_n<U>(P, eq, kq);
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      const uint32_t e = e0 + 64 * q + lane;
-      const int k = lane_search(P, e);
-      kq[q] = k;
-      idx[q] = static_cast<uint32_t>(__shfl(cbP, k)) + e;
-      a[q] = e < T ? adj[idx[q]] : -1;
+      idx[q] = static_cast<uint32_t>(__shfl(cbP, kq[q])) + eq[q];
+      a[q] = eq[q] < T ? adj[idx[q]] : -1;
     }
 #pragma unroll
     for (int q = 0; q < U; q++) {

@@ -4,20 +4,38 @@
 //   (set_intersection_GPU_bitmap_warp_per_vertex,
 //    scan-query-cuda/scan_xp_multi_gpu_multi_pass_common.cu:308-407):
 //   for each source u a membership set of N(u) is built once, then every
-//   neighbor list N(v), v > u, is probed against it.  Sets of at most
-//   kLdsCap ids live in an LDS hash table; hub sources use an n-bit bitmap
-//   in HBM owned by one workgroup of a persistent launch (the reference's
-//   per-SM bitmap pool, without its %smid slot search).
+//   neighbor list N(v), v > u, is probed against it.  Small sets (up to
+//   ACC_PARAM_BMP_HASH_MAX ids, 256 by default) live in an LDS hash table,
+//   behind an LDS index of N(u) (one bit per 32 ids) in a workgroup unit;
+//   larger sets spill to an n-bit bitmap in HBM,
+//   built for a batch of hubs by one kernel and shared by every workgroup
+//   that probes that hub, with an LDS index of one bit per bitmap word (per
+//   group of words beyond 2^17 words) checked before the HBM word, the
+//   reference's BITMAP_SCALE_LOG index level (same file :320-382).  LDS hash
+//   tables of 2^12, 2^14 and 2^15 keys (up to 16384 ids; gfx950 gives a
+//   workgroup 160 KB of LDS) remain selectable for larger sets, though they
+//   probe slower than the indexed HBM bitmaps (at R-MAT scale 22: 103 ms for
+//   the whole path with tables up to 2048 ids, 80 ms with 256).
 // ACC_PATH_MERGE -- the reference's MPS kernels
 //   (set_intersection_GPU_shared_merge_filtered / _galloping_filtered, same
-//    file :68-176): one wave per edge; balanced pairs merge the shorter list
-//   against 64-element LDS-staged windows of the longer one with a binary
-//   search per lane and a ballot/popcount; skewed pairs (ratio above
-//   kSkew) binary-search the longer list in global memory per element.
+//    file :32-176): N(u) is staged in LDS in windows of sorted ids; every
+//   element of a probed list N(v) finds its rank in the window by a binary
+//   search there (the merge-path co-rank; a skewed pair costs log d_u per
+//   element of the shorter list, a balanced pair log d_u too, with no
+//   global-memory search), and a window's slice of N(v) is found by one
+//   binary search per window boundary.
 //
-// Both write 2 + |N(u) ∩ N(v)| (or the raw count) into the upper slot (u, v)
-// and its reverse slot (the position of u in N(v)), with no atomics on the
-// counts, and add the raw counts to the triangle sum.
+// Both paths share the probe layout of the forward path: a workgroup unit is
+// one source u and at most kChunk of its upper slots (u, v); the lists N(v)
+// (or their slices) are laid end to end, element e of the unit's range
+// belongs to probe k(e) = max{k : P_k <= e} over the exclusive prefix P of
+// the lengths (a 9-step search in LDS), each lane takes one element per
+// step, so no lane idles on a short list, and per-probe hit counts go to
+// LDS counters (one popcount when a 64-element round lies in one probe).
+// Each unit then writes 2 + |N(u) ∩ N(v)| (or the raw count) into the upper
+// slot (u, v) and its reverse slot (from the reverse-slot index built at
+// upload), with no atomics on the counts, and adds the raw counts to the
+// triangle sum.
 
 #include "acc_internal.h"
 
@@ -25,259 +43,527 @@ namespace {
 
 constexpr int kWG = accfwd::kWG;
 constexpr int kWaves = accfwd::kWaves;
-constexpr int kTBits = accfwd::kTableBits;
-constexpr int kT = 1 << kTBits;
 constexpr uint32_t kChunk = accfwd::kChunk;
-constexpr uint32_t kSkew = 32;
+constexpr int kU = 4;  // 64-element rounds per wave step: loads in flight per lane
+constexpr int kMergeU = 8;  // the same for the merge path's large windows (13-step LDS searches)
+
+static_assert(kWG == kChunk, "one probe per thread of the unit");
 
 __device__ __forceinline__ uint32_t hslot(int32_t a, int bits) {
   return (static_cast<uint32_t>(a) * 0x9E3779B1u) >> (32 - bits);
 }
 
-__device__ __forceinline__ int wsum(int v) {
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
+  uint32_t x = v;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x - v;
 }
 
-__device__ __forceinline__ bool hash_has(const int32_t *keys, int bits, uint32_t mask, int32_t a) {
-  uint32_t h = hslot(a, bits);
-  for (;;) {
-    const int32_t k = keys[h];
-    if (k == a) return true;
-    if (k == -1) return false;
-    h = (h + 1) & mask;
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ void lds_add(int32_t *p, int v) {
+  __hip_atomic_fetch_add((lds_i32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_or(uint32_t *p, uint32_t v) {
+  __hip_atomic_fetch_or((lds_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Adds the round's hits to the probe counters.  The probe index k is
+// nondecreasing over the lanes, so the lanes fall into segments of equal k:
+// the first lane of each segment adds the popcount of the segment's hits (one
+// add in all when the whole round lies in one probe).
+__device__ __forceinline__ void count_round(int32_t *ctr, int k, bool hit, int lane) {
+  const unsigned long long hits = __ballot(hit);
+  if (!hits) return;
+  const int k0 = __builtin_amdgcn_readfirstlane(k);
+  if (k0 == __builtin_amdgcn_readlane(k, 63)) {
+    if (lane == 0) lds_add(&ctr[k0], static_cast<int>(__popcll(hits)));
+    return;
+  }
+  const bool head = lane == 0 || k != __shfl_up(k, 1, 64);
+  const unsigned long long heads = __ballot(head);
+  if (head) {
+    const unsigned long long above = heads & ~((2ull << lane) - 1);  // segment heads after this lane
+    const unsigned long long seg = (above ? (above & (0ull - above)) - 1 : ~0ull) & ~((1ull << lane) - 1);
+    const int c = static_cast<int>(__popcll(hits & seg));
+    if (c) lds_add(&ctr[k], c);
   }
 }
 
-// Position of x in the sorted list adj[b, e) (x must be present); all lanes
-// run the same search, so the loads are broadcasts.
-__device__ __forceinline__ uint32_t find_pos(const int32_t *__restrict__ adj, uint32_t b, uint32_t e, int32_t x) {
-  while (b < e) {
-    const uint32_t mid = b + ((e - b) >> 1);
-    if (adj[mid] < x)
-      b = mid + 1;
-    else
-      e = mid;
-  }
-  return b;
-}
+// Probe metadata of a unit in LDS.
+struct UnitLds {
+  uint32_t P[kChunk];    // exclusive prefix of the probe lengths; T beyond the probes
+  uint32_t cb[kChunk];   // first slot of each probe less its prefix
+  int32_t ctr[kChunk];   // hits per probe
+  uint32_t wsum[kWaves];
+};
 
-__device__ __forceinline__ void block_add_sum(unsigned long long *sum, unsigned long long s) {
-  __shared__ unsigned long long part[kWaves];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+// Lays the unit's probes end to end: thread t < nup passes its probe's first
+// slot cb and length len (others len 0).  Returns the total T (uniform).
+// Callers synchronise before reading P and cb.
+__device__ __forceinline__ uint32_t flatten(UnitLds &L, uint32_t cb, uint32_t len, uint32_t nup) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t excl = wave_excl_scan(len, lane);
+  if (lane == 63) L.wsum[wave] = excl + len;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < kWaves; w++) t += part[w];
-    if (t) atomicAdd(sum, t);
+  uint32_t before = 0, T = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; w++) {
+    const uint32_t ws = L.wsum[w];
+    before += w < wave ? ws : 0;
+    T += ws;
+  }
+  L.P[tid] = static_cast<uint32_t>(tid) < nup ? before + excl : T;
+  if (static_cast<uint32_t>(tid) < nup) L.cb[tid] = cb - (before + excl);
+  return __builtin_amdgcn_readfirstlane(T);
+}
+
+// Probes the T flattened elements against set.  Wave w takes the contiguous
+// range [w R, (w + 1) R) of the elements (R a multiple of 64, the ranges
+// equal but for the last), in steps of kU * 64.  Each step first advances a
+// wave-uniform probe index kc to the probe of its first element (one LDS
+// read while that probe goes on, a binary search of P over [kc, 512)
+// otherwise); when the step ends inside probe kc, every lane takes kc, else
+// each lane finds its element's probe by a 9-step search of P.  Step indices
+// and kc are wave-uniform.
+template <int U = kU, class Set>
+__device__ __forceinline__ void probe_flat(const int32_t *__restrict__ adj, const Set &set, UnitLds &L, uint32_t T) {
+  static_assert(kChunk == 512, "the search below covers 512 entries");
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t R = ((T + kWaves - 1) / kWaves + 63) & ~63u;
+  const uint32_t lo = min(T, static_cast<uint32_t>(wave) * R), hi = min(T, lo + R);
+  int kc = 0;
+  for (uint32_t e0 = lo; e0 < hi; e0 += 64 * U) {
+    if (kc < static_cast<int>(kChunk) - 1 && L.P[kc + 1] <= e0) {
+      int lo_k = kc + 1, hi_k = static_cast<int>(kChunk) - 1;  // the largest k with P[k] <= e0 lies in [lo_k, hi_k]
+      while (lo_k < hi_k) {
+        const int mid = (lo_k + hi_k + 1) >> 1;
+        if (L.P[mid] <= e0)
+          lo_k = mid;
+        else
+          hi_k = mid - 1;
+      }
+      kc = lo_k;
+    }
+    kc = __builtin_amdgcn_readfirstlane(kc);
+    const uint32_t last = min(e0 + 64 * U, hi) - 1;
+    const bool one = kc == static_cast<int>(kChunk) - 1 || L.P[kc + 1] > last;
+    int32_t a[U];
+    int kq[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) kq[q] = kc;
+    if (!one) {
+      // the U searches advance together, one LDS read of each per step
+#pragma unroll
+      for (int q = 0; q < U; q++) kq[q] = 0;
+#pragma unroll
+      for (int st = 256; st; st >>= 1) {
+        uint32_t p[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) p[q] = L.P[kq[q] + st];
+#pragma unroll
+        for (int q = 0; q < U; q++)
+          if (p[q] <= e0 + 64 * q + lane) kq[q] += st;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t e = e0 + 64 * q + lane;
+      a[q] = e < hi ? adj[L.cb[kq[q]] + e] : -1;
+    }
+    bool hit[U];
+    set.template has_n<U>(a, hit);
+#pragma unroll
+    for (int q = 0; q < U; q++) count_round(L.ctr, kq[q], hit[q], lane);
   }
 }
 
-// Stores the count of edge (u, v) into its upper slot j and reverse slot p.
-__device__ __forceinline__ void store_pair(int32_t *cnt, uint32_t j, uint32_t p, int c, int plus) {
-  cnt[j] = c + plus;
-  cnt[p] = c + plus;
+// Writes each probe's count into its upper slot and reverse slot and adds
+// the unit's raw counts to *sum.  Follows a __syncthreads.
+__device__ __forceinline__ void finish_unit(const UnitLds &L, const uint32_t *__restrict__ rev, uint32_t j0,
+                                            uint32_t nup, int32_t *cnt, int plus, unsigned long long *sum) {
+  const int tid = threadIdx.x;
+  int c = 0;
+  if (static_cast<uint32_t>(tid) < nup) {
+    c = L.ctr[tid];
+    cnt[j0 + tid] = c + plus;
+    const uint32_t p = rev[j0 + tid];
+    if (p != ~0u) cnt[p] = c + plus;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((tid & 63) == 0 && c) atomicAdd(sum, static_cast<unsigned long long>(c));
+}
+
+// The probe of upper slot j: N(v) for v = adj[j] (empty for an id out of range).
+__device__ __forceinline__ void probe_of(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj, uint32_t n,
+                                         uint32_t j, uint32_t &cb, uint32_t &ce) {
+  const int32_t v = adj[j];
+  if (v >= 0 && static_cast<uint32_t>(v) < n) {
+    cb = off[v];
+    ce = off[v + 1];
+  } else {
+    cb = ce = 0;
+  }
+}
+
+// ---------------------------------------------------------------- sets
+// Open-addressing hash set of N(u) in LDS (keys only; -1 = empty).
+struct LdsHash {
+  const int32_t *keys;
+  int bits;
+  uint32_t mask;
+  __device__ __forceinline__ bool has(int32_t a) const {
+    uint32_t h = hslot(a, bits);
+    for (;;) {
+      const int32_t k = keys[h];
+      if (k == a) return true;
+      if (k == -1) return false;
+      h = (h + 1) & mask;
+    }
+  }
+  // U lookups (a < 0: none): the first probe of each together, then the
+  // rare collision chains one by one.
+  template <int U>
+  __device__ __forceinline__ void has_n(const int32_t (&a)[U], bool (&hit)[U]) const {
+    uint32_t h[U];
+    int32_t k[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      h[q] = hslot(a[q] < 0 ? 0 : a[q], bits);
+      k[q] = keys[h[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      hit[q] = a[q] >= 0 && k[q] == a[q];
+      if (a[q] >= 0 && k[q] != a[q] && k[q] != -1) hit[q] = has(a[q]);
+    }
+  }
+};
+
+// The same table behind an LDS index with one bit per 2^shift words of 32
+// ids (set when N(u) holds one of their ids): most probes of a small set miss
+// the index and never walk the table.
+struct IndexedHash {
+  const uint32_t *idx;
+  int shift;
+  LdsHash h;
+  template <int U>
+  __device__ __forceinline__ void has_n(const int32_t (&a)[U], bool (&hit)[U]) const {
+    int32_t b[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t iw = (static_cast<uint32_t>(a[q] < 0 ? 0 : a[q]) >> 5) >> shift;
+      b[q] = a[q] >= 0 && ((idx[iw >> 5] >> (iw & 31)) & 1u) ? a[q] : -1;
+    }
+    h.has_n<U>(b, hit);
+  }
+};
+
+// N(u) of a hub as a bitmap in HBM, behind an LDS index with one bit per
+// 2^shift bitmap words (set when any of them holds a bit).
+struct HubBits {
+  const uint32_t *idx;
+  const uint32_t *__restrict__ bm;
+  int shift;
+  template <int U>
+  __device__ __forceinline__ void has_n(const int32_t (&a)[U], bool (&hit)[U]) const {
+    bool in[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t iw = (static_cast<uint32_t>(a[q] < 0 ? 0 : a[q]) >> 5) >> shift;
+      in[q] = a[q] >= 0 && ((idx[iw >> 5] >> (iw & 31)) & 1u);
+    }
+    uint32_t w[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) w[q] = in[q] ? bm[static_cast<uint32_t>(a[q]) >> 5] : 0u;
+#pragma unroll
+    for (int q = 0; q < U; q++) hit[q] = in[q] && ((w[q] >> (a[q] & 31)) & 1u);
+  }
+};
+
+// A window of sorted ids of N(u) in LDS; membership by a branch-free binary
+// search (top = the largest power of two <= len).
+struct SortedWin {
+  const int32_t *s;
+  uint32_t len;
+  uint32_t top;
+  // U searches advanced together, one LDS read of each per step.
+  template <int U>
+  __device__ __forceinline__ void has_n(const int32_t (&a)[U], bool (&hit)[U]) const {
+    uint32_t pos[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) pos[q] = 0;
+    for (uint32_t step = top; step; step >>= 1) {
+      int32_t v[U];
+#pragma unroll
+      for (int q = 0; q < U; q++) v[q] = s[min(pos[q] + step, len) - 1];
+#pragma unroll
+      for (int q = 0; q < U; q++)
+        if (pos[q] + step <= len && v[q] < a[q]) pos[q] += step;
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) hit[q] = a[q] >= 0 && pos[q] < len && s[min(pos[q], len - 1)] == a[q];
+  }
+};
+
+// ---------------------------------------------------------------- wave units
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave units: one source u (d_u <= kEdgeWaveDeg, at most 64 upper slots) per
+// wave, eight per workgroup, with no workgroup barrier: lane t holds probe t
+// (upper slot j0 + t), the wave lays the probes end to end and finds each
+// element's probe by a 6-step search over the lanes' prefix.  N(u) is a
+// wave-private LDS hash table (bitmap path) or sorted run (merge path).
+template <bool MERGE>
+__global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ rev, uint32_t n,
+                                                   const FwdUnit *__restrict__ units, uint32_t n_units, int32_t *cnt,
+                                                   int plus, unsigned long long *sum) {
+  constexpr int kT = 2 * kEdgeWaveDeg;
+  __shared__ int32_t tab[kWaves][kT];
+  __shared__ int32_t ctrs[kWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t ui = blockIdx.x * kWaves + static_cast<uint32_t>(wave);
+  int c = 0;
+  if (ui < n_units) {
+    const FwdUnit un = units[ui];
+    const int32_t u = un.b;
+    const uint32_t su = off[u], du = off[u + 1] - su;
+    const uint32_t nup = un.j1 - un.j0;
+    int32_t *t = tab[wave];
+    int32_t *ct = ctrs[wave];
+    uint32_t cb = 0, ce = 0;
+    if (static_cast<uint32_t>(lane) < nup) probe_of(off, adj, n, un.j0 + lane, cb, ce);
+    const uint32_t len = ce - cb;
+    const uint32_t P = wave_excl_scan(len, lane);
+    const uint32_t T = __builtin_amdgcn_readfirstlane(__shfl(P + len, 63));
+    ct[lane] = 0;
+    int bits = 6;
+    while ((1u << bits) < 2 * du) bits++;
+    const uint32_t mask = (1u << bits) - 1;
+    uint32_t top = 1;
+    while (top * 2 <= du) top *= 2;
+    if (MERGE) {
+      for (uint32_t i = lane; i < du; i += 64) t[i] = adj[su + i];
+    } else {
+      for (uint32_t i = lane; i <= mask; i += 64) t[i] = -1;
+      wave_lds_sync();
+      for (uint32_t i = lane; i < du; i += 64) {
+        const int32_t a = adj[su + i];
+        uint32_t h = hslot(a, bits);
+        while (atomicCAS(&t[h], -1, a) != -1) h = (h + 1) & mask;
+      }
+    }
+    wave_lds_sync();
+    for (uint32_t e0 = 0; e0 < T; e0 += 64 * kU) {
+      int32_t a[kU];
+      int kq[kU];
+#pragma unroll
+      for (int q = 0; q < kU; q++) kq[q] = 0;
+#pragma unroll
+      for (int st = 32; st; st >>= 1) {  // the kU lane searches advance together
+        uint32_t v[kU];
+#pragma unroll
+        for (int q = 0; q < kU; q++) v[q] = static_cast<uint32_t>(__shfl(P, kq[q] + st));
+#pragma unroll
+        for (int q = 0; q < kU; q++) {
+          const uint32_t e = e0 + 64 * q + lane;
+          if (v[q] <= (e < T ? e : T - 1)) kq[q] += st;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kU; q++) {
+        const uint32_t e = e0 + 64 * q + lane;
+        const uint32_t pk = __shfl(P, kq[q]), ck = __shfl(cb, kq[q]);
+        a[q] = e < T ? adj[ck + (e - pk)] : -1;
+      }
+      bool hit[kU];
+      if (MERGE)
+        SortedWin{t, du, top}.has_n<kU>(a, hit);
+      else
+        LdsHash{t, bits, mask}.has_n<kU>(a, hit);
+#pragma unroll
+      for (int q = 0; q < kU; q++) count_round(ct, kq[q], hit[q], lane);
+    }
+    wave_lds_sync();
+    if (static_cast<uint32_t>(lane) < nup) {
+      c = ct[lane];
+      cnt[un.j0 + lane] = c + plus;
+      const uint32_t p = rev[un.j0 + lane];
+      if (p != ~0u) cnt[p] = c + plus;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if (lane == 0 && c) atomicAdd(sum, static_cast<unsigned long long>(c));
 }
 
 // ---------------------------------------------------------------- BITMAP path
-// LDS-set units: one workgroup per chunk of at most kChunk upper slots of a
-// source u with d_u <= kCap.
-__global__ __launch_bounds__(kWG) void k_bmp_lds(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                                 const FwdUnit *__restrict__ units, int32_t *cnt, int plus,
-                                                 unsigned long long *sum) {
-  __shared__ int32_t keys[kT];
-  __shared__ uint32_t s_cb[kChunk], s_ce[kChunk];
+// LDS-hash units: one workgroup per chunk of at most kChunk upper slots of a
+// source u with d_u <= 2^(TBITS-1); the table has the smallest power of two
+// >= 2 d_u entries (at least 64).
+// With INDEX, an LDS index of N(u) (one bit per 2^shift words of 32 ids,
+// kIdxWords words) filters the probes first.
+constexpr uint32_t kIdxWords = 4096;  // LDS index of 2^17 bits
+template <int TBITS, bool INDEX>
+__global__ __launch_bounds__(kWG) void k_bmp_hash(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                  const uint32_t *__restrict__ rev, uint32_t n,
+                                                  const FwdUnit *__restrict__ units, uint64_t words, int shift,
+                                                  int32_t *cnt, int plus, unsigned long long *sum) {
+  __shared__ int32_t keys[1 << TBITS];
+  __shared__ uint32_t idx[INDEX ? kIdxWords : 1];
+  __shared__ UnitLds L;
   const FwdUnit un = units[blockIdx.x];
   const int32_t u = un.b;
   const uint32_t su = off[u], du = off[u + 1] - su;
   const uint32_t nup = un.j1 - un.j0;
+  const int tid = threadIdx.x;
   int bits = 6;
   while ((1u << bits) < 2 * du) bits++;
   const uint32_t mask = (1u << bits) - 1;
-  for (uint32_t i = threadIdx.x; i <= mask; i += kWG) keys[i] = -1;
-  for (uint32_t i = threadIdx.x; i < nup; i += kWG) {
-    const int32_t v = adj[un.j0 + i];
-    s_cb[i] = off[v];
-    s_ce[i] = off[v + 1];
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < du; i += kWG) {
+  for (uint32_t i = tid; i <= mask; i += kWG) keys[i] = -1;
+  const uint32_t iwords = static_cast<uint32_t>((((words + (1ull << shift) - 1) >> shift) + 31) / 32);
+  if (INDEX)
+    for (uint32_t i = tid; i < iwords; i += kWG) idx[i] = 0;
+  uint32_t cb = 0, ce = 0;
+  if (static_cast<uint32_t>(tid) < nup) probe_of(off, adj, n, un.j0 + tid, cb, ce);
+  L.ctr[tid] = 0;
+  const uint32_t T = flatten(L, cb, ce - cb, nup);
+  for (uint32_t i = tid; i < du; i += kWG) {
     const int32_t a = adj[su + i];
     uint32_t h = hslot(a, bits);
     while (atomicCAS(&keys[h], -1, a) != -1) h = (h + 1) & mask;
+    if (INDEX) {
+      const uint32_t iw = (static_cast<uint32_t>(a) >> 5) >> shift;
+      lds_or(&idx[iw >> 5], 1u << (iw & 31));
+    }
   }
   __syncthreads();
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  unsigned long long s = 0;
-  for (uint32_t k = wave; k < nup; k += kWaves) {
-    const uint32_t vb = __builtin_amdgcn_readfirstlane(s_cb[k]);
-    const uint32_t ve = __builtin_amdgcn_readfirstlane(s_ce[k]);
-    int hits = 0;
-    uint32_t p = ~0u;
-    for (uint32_t base = vb; base < ve; base += 64) {
-      const uint32_t i = base + lane;
-      const int32_t a = i < ve ? adj[i] : -1;
-      if (a >= 0 && hash_has(keys, bits, mask, a)) hits++;
-      const unsigned long long mu = __ballot(a == u);
-      if (mu) p = base + static_cast<uint32_t>(__builtin_ctzll(mu));
-    }
-    const int c = wsum(hits);
-    if (lane == 0) {
-      s += static_cast<unsigned long long>(c);
-      if (p < ve) store_pair(cnt, un.j0 + k, p, c, plus);
-    }
-  }
-  block_add_sum(sum, s);
+  if (INDEX)
+    probe_flat(adj, IndexedHash{idx, shift, LdsHash{keys, bits, mask}}, L, T);
+  else
+    probe_flat(adj, LdsHash{keys, bits, mask}, L, T);
+  __syncthreads();
+  finish_unit(L, rev, un.j0, nup, cnt, plus, sum);
 }
 
-// Hub units: persistent workgroups; workgroup w owns bitmap w of the pool
-// (words = ceil(n/32) each, zero between units) and takes units w, w+G, ...
-__global__ __launch_bounds__(kWG) void k_bmp_hub(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                                 const FwdUnit *__restrict__ units, uint32_t n_units,
-                                                 uint32_t *__restrict__ pool, uint64_t words, int32_t *cnt, int plus,
-                                                 unsigned long long *sum) {
-  __shared__ uint32_t s_cb[kChunk], s_ce[kChunk];
+// Hub bitmaps of one batch: workgroup h sets the bits of N(hubs[h]) in
+// bitmap h of the pool (zero before), or clears them again (set == false).
+__global__ __launch_bounds__(256) void k_hub_bits(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                  const uint32_t *__restrict__ hubs, uint32_t *pool, uint64_t words,
+                                                  bool set) {
+  const uint32_t u = hubs[blockIdx.x];
   uint32_t *bm = pool + static_cast<uint64_t>(blockIdx.x) * words;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  unsigned long long s = 0;
-  for (uint32_t ui = blockIdx.x; ui < n_units; ui += gridDim.x) {
-    const FwdUnit un = units[ui];
-    const int32_t u = un.b;
-    const uint32_t su = off[u], eu = off[u + 1];
-    const uint32_t nup = un.j1 - un.j0;
-    for (uint32_t i = threadIdx.x; i < nup; i += kWG) {
-      const int32_t v = adj[un.j0 + i];
-      s_cb[i] = off[v];
-      s_ce[i] = off[v + 1];
-    }
-    for (uint32_t i = su + threadIdx.x; i < eu; i += kWG) {
-      const int32_t a = adj[i];
-      atomicOr(&bm[a >> 5], 1u << (a & 31));
-    }
-    __threadfence();
-    __syncthreads();
-    for (uint32_t k = wave; k < nup; k += kWaves) {
-      const uint32_t vb = __builtin_amdgcn_readfirstlane(s_cb[k]);
-      const uint32_t ve = __builtin_amdgcn_readfirstlane(s_ce[k]);
-      int hits = 0;
-      uint32_t p = ~0u;
-      for (uint32_t base = vb; base < ve; base += 64) {
-        const uint32_t i = base + lane;
-        const int32_t a = i < ve ? adj[i] : -1;
-        if (a >= 0) {
-          // atomic load: the bitmap was written by other waves through L2
-          const uint32_t w = __hip_atomic_load(&bm[a >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          hits += (w >> (a & 31)) & 1u;
-        }
-        const unsigned long long mu = __ballot(a == u);
-        if (mu) p = base + static_cast<uint32_t>(__builtin_ctzll(mu));
-      }
-      const int c = wsum(hits);
-      if (lane == 0) {
-        s += static_cast<unsigned long long>(c);
-        if (p < ve) store_pair(cnt, un.j0 + k, p, c, plus);
-      }
-    }
-    __syncthreads();
-    for (uint32_t i = su + threadIdx.x; i < eu; i += kWG)
-      __hip_atomic_store(&bm[adj[i] >> 5], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    __syncthreads();
+  for (uint32_t i = off[u] + threadIdx.x; i < off[u + 1]; i += 256) {
+    const int32_t a = adj[i];
+    if (set)
+      atomicOr(&bm[static_cast<uint32_t>(a) >> 5], 1u << (a & 31));
+    else
+      bm[static_cast<uint32_t>(a) >> 5] = 0;
   }
-  block_add_sum(sum, s);
+}
+
+// Hub units of one batch: the unit's source is hub slot un.nlow of the pool.
+__global__ __launch_bounds__(kWG) void k_bmp_hub(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                 const uint32_t *__restrict__ rev, uint32_t n,
+                                                 const FwdUnit *__restrict__ units, const uint32_t *__restrict__ pool,
+                                                 uint64_t words, int shift, int32_t *cnt, int plus,
+                                                 unsigned long long *sum) {
+  __shared__ uint32_t idx[kIdxWords];
+  __shared__ UnitLds L;
+  const FwdUnit un = units[blockIdx.x];
+  const int32_t u = un.b;
+  const uint32_t su = off[u], eu = off[u + 1];
+  const uint32_t nup = un.j1 - un.j0;
+  const int tid = threadIdx.x;
+  const uint32_t iwords = static_cast<uint32_t>((((words + (1ull << shift) - 1) >> shift) + 31) / 32);
+  for (uint32_t i = tid; i < iwords; i += kWG) idx[i] = 0;
+  uint32_t cb = 0, ce = 0;
+  if (static_cast<uint32_t>(tid) < nup) probe_of(off, adj, n, un.j0 + tid, cb, ce);
+  L.ctr[tid] = 0;
+  const uint32_t T = flatten(L, cb, ce - cb, nup);
+  for (uint32_t i = su + tid; i < eu; i += kWG) {
+    const uint32_t iw = (static_cast<uint32_t>(adj[i]) >> 5) >> shift;
+    lds_or(&idx[iw >> 5], 1u << (iw & 31));
+  }
+  __syncthreads();
+  probe_flat(adj, HubBits{idx, pool + static_cast<uint64_t>(un.nlow) * words, shift}, L, T);
+  __syncthreads();
+  finish_unit(L, rev, un.j0, nup, cnt, plus, sum);
 }
 
 // ---------------------------------------------------------------- MERGE path
-// One wave per upper slot of the unit list; 8 waves per workgroup round-robin.
-__global__ __launch_bounds__(kWG) void k_merge(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
-                                               const FwdUnit *__restrict__ units, int32_t *cnt, int plus,
-                                               unsigned long long *sum) {
-  __shared__ int32_t win[kWaves][64];
+// Windowed units: N(u) in windows of 2^WBITS sorted ids staged in LDS; per
+// window w, probe k's slice of N(v) holds the ids in [N(u)[w W], N(u)[(w+1) W])
+// (unbounded below for the first window and above for the last), found by a
+// binary search of N(v) for the upper boundary, the lower being the previous
+// window's.  Every element of N(v) lies in exactly one slice.
+template <int WBITS, int U>
+__global__ __launch_bounds__(kWG) void k_merge_win(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
+                                                   const uint32_t *__restrict__ rev, uint32_t n,
+                                                   const FwdUnit *__restrict__ units, int32_t *cnt, int plus,
+                                                   unsigned long long *sum) {
+  constexpr uint32_t W = 1u << WBITS;
+  __shared__ int32_t win[W];
+  __shared__ UnitLds L;
   const FwdUnit un = units[blockIdx.x];
   const int32_t u = un.b;
-  const uint32_t ub = off[u], ue = off[u + 1];
+  const uint32_t su = off[u], du = off[u + 1] - su;
   const uint32_t nup = un.j1 - un.j0;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  int32_t *w = win[wave];
-  unsigned long long s = 0;
-  for (uint32_t k = wave; k < nup; k += kWaves) {
-    const uint32_t j = un.j0 + k;
-    const int32_t v = adj[j];
-    const uint32_t vb = off[v], ve = off[v + 1];
-    // A: shorter list, B: longer list
-    uint32_t ab = ub, ae = ue, bb = vb, be = ve;
-    if (ae - ab > be - bb) {
-      ab = vb;
-      ae = ve;
-      bb = ub;
-      be = ue;
-    }
-    const uint32_t la = ae - ab, lb = be - bb;
-    int hits = 0;
-    if (la == 0) {
-      // no common neighbor
-    } else if (lb > kSkew * la) {
-      // skewed: binary search of each A element in B
-      for (uint32_t i0 = ab; i0 < ae; i0 += 64) {
-        const uint32_t i = i0 + lane;
-        if (i < ae) {
-          const int32_t x = adj[i];
-          const uint32_t q = find_pos(adj, bb, be, x);
-          hits += (q < be && adj[q] == x) ? 1 : 0;
-        }
+  const int tid = threadIdx.x;
+  uint32_t cur = 0, ce = 0;  // this thread's probe: the unprobed rest [cur, ce) of N(v)
+  if (static_cast<uint32_t>(tid) < nup) probe_of(off, adj, n, un.j0 + tid, cur, ce);
+  L.ctr[tid] = 0;
+  const uint32_t nwin = (du + W - 1) / W;
+  for (uint32_t w = 0; w < nwin; w++) {
+    const uint32_t w0 = w * W, wl = du - w0 < W ? du - w0 : W;
+    for (uint32_t i = tid; i < wl; i += kWG) win[i] = adj[su + w0 + i];
+    // slice end: the first element of N(v) at or above the next window's first id
+    uint32_t end = ce;
+    if (w + 1 < nwin && cur < ce) {
+      const int32_t bound = adj[su + w0 + W];
+      uint32_t lo = cur, hi = ce;
+      while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (adj[mid] < bound)
+          lo = mid + 1;
+        else
+          hi = mid;
       }
-    } else {
-      // windowed merge: A chunk in registers, B window staged in LDS
-      uint32_t ia = ab, ib = bb;
-      while (ia < ae && ib < be) {
-        const int32_t x = ia + lane < ae ? adj[ia + lane] : INT32_MAX;
-        const int32_t y = ib + lane < be ? adj[ib + lane] : INT32_MAX;
-        w[lane] = y;
-        const uint32_t na = ae - ia < 64 ? ae - ia : 64;
-        const uint32_t nb = be - ib < 64 ? be - ib : 64;
-        const int32_t amax = __shfl(x, static_cast<int>(na - 1), 64);
-        const int32_t bmax = __shfl(y, static_cast<int>(nb - 1), 64);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (x != INT32_MAX) {
-          int lo = 0, hi = static_cast<int>(nb);
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (w[mid] < x)
-              lo = mid + 1;
-            else
-              hi = mid;
-          }
-          hits += (lo < static_cast<int>(nb) && w[lo] == x) ? 1 : 0;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int32_t am = __builtin_amdgcn_readfirstlane(amax);
-        const int32_t bm = __builtin_amdgcn_readfirstlane(bmax);
-        if (am <= bm) ia += 64;
-        if (bm <= am) ib += 64;
-      }
+      end = lo;
     }
-    const int c = wsum(hits);
-    const uint32_t p = find_pos(adj, vb, ve, u);
-    if (lane == 0) {
-      s += static_cast<unsigned long long>(c);
-      if (p < ve) store_pair(cnt, j, p, c, plus);
-    }
+    const uint32_t T = flatten(L, cur, end - cur, nup);
+    __syncthreads();
+    uint32_t top = 1;
+    while (top * 2 <= wl) top *= 2;
+    probe_flat<U>(adj, SortedWin{win, wl, top}, L, T);
+    cur = end;
+    __syncthreads();
   }
-  block_add_sum(sum, s);
+  finish_unit(L, rev, un.j0, nup, cnt, plus, sum);
 }
 
 __global__ void k_zero_sum(unsigned long long *sum) {
   if (threadIdx.x == 0) *sum = 0;
+}
+
+template <class K, class... A>
+int launch_units(K kernel, uint32_t units, hipStream_t st, A... args) {
+  if (!units) return ACC_OK;
+  hipLaunchKernelGGL(kernel, dim3(units), dim3(kWG), 0, st, args...);
+  ACC_HIP_TRY(hipGetLastError());
+  return ACC_OK;
 }
 
 }  // namespace
@@ -286,15 +572,29 @@ namespace accpaths {
 
 int launch_bitmap(acc_graph *g, int plus) {
   hipStream_t st = g->ctx->stream;
+  const EdgePlan &p = g->edge;
   hipLaunchKernelGGL(k_zero_sum, dim3(1), dim3(64), 0, st, g->d_sum);
-  if (g->n_bmp_lds) {
-    hipLaunchKernelGGL(k_bmp_lds, dim3(g->n_bmp_lds), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_bmp_lds, g->d_cnt,
-                       plus, g->d_sum);
+  ACC_HIP_TRY(hipGetLastError());
+  ACC_TRY(launch_units(k_edge_wave<false>, (p.n_units[kEdgeWave] + kWaves - 1) / kWaves, st, g->d_off, g->d_adj,
+                       g->d_rev, g->n, p.d_units[kEdgeWave], p.n_units[kEdgeWave], g->d_cnt, plus, g->d_sum));
+  ACC_TRY(launch_units(k_bmp_hash<12, true>, p.n_units[kEdgeHash12], st, g->d_off, g->d_adj, g->d_rev, g->n,
+                       p.d_units[kEdgeHash12], p.pool_words, p.idx_shift, g->d_cnt, plus, g->d_sum));
+  ACC_TRY(launch_units(k_bmp_hash<14, false>, p.n_units[kEdgeHash14], st, g->d_off, g->d_adj, g->d_rev, g->n,
+                       p.d_units[kEdgeHash14], p.pool_words, p.idx_shift, g->d_cnt, plus, g->d_sum));
+  ACC_TRY(launch_units(k_bmp_hash<15, false>, p.n_units[kEdgeHash15], st, g->d_off, g->d_adj, g->d_rev, g->n,
+                       p.d_units[kEdgeHash15], p.pool_words, p.idx_shift, g->d_cnt, plus, g->d_sum));
+  // hub batches: set the batch's bitmaps, probe its units, clear the bitmaps
+  for (size_t k = 0; k + 1 < p.hub_batch.size(); k++) {
+    const uint32_t h0 = p.hub_batch[k], h1 = p.hub_batch[k + 1];
+    const uint32_t u0 = p.hub_unit_batch[k], u1 = p.hub_unit_batch[k + 1];
+    if (h1 == h0) continue;
+    hipLaunchKernelGGL(k_hub_bits, dim3(h1 - h0), dim3(256), 0, st, g->d_off, g->d_adj, p.d_hubs + h0, p.d_pool,
+                       p.pool_words, true);
     ACC_HIP_TRY(hipGetLastError());
-  }
-  if (g->n_bmp_hub) {
-    hipLaunchKernelGGL(k_bmp_hub, dim3(g->bmp_pool_slots), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_bmp_hub,
-                       g->n_bmp_hub, g->d_bmp_pool, g->bmp_pool_words, g->d_cnt, plus, g->d_sum);
+    ACC_TRY(launch_units(k_bmp_hub, u1 - u0, st, g->d_off, g->d_adj, g->d_rev, g->n, p.d_units[kEdgeHub] + u0,
+                         p.d_pool, p.pool_words, p.idx_shift, g->d_cnt, plus, g->d_sum));
+    hipLaunchKernelGGL(k_hub_bits, dim3(h1 - h0), dim3(256), 0, st, g->d_off, g->d_adj, p.d_hubs + h0, p.d_pool,
+                       p.pool_words, false);
     ACC_HIP_TRY(hipGetLastError());
   }
   return ACC_OK;
@@ -302,12 +602,18 @@ int launch_bitmap(acc_graph *g, int plus) {
 
 int launch_merge(acc_graph *g, int plus) {
   hipStream_t st = g->ctx->stream;
+  const EdgePlan &p = g->edge;
   hipLaunchKernelGGL(k_zero_sum, dim3(1), dim3(64), 0, st, g->d_sum);
-  if (g->n_edge_units) {
-    hipLaunchKernelGGL(k_merge, dim3(g->n_edge_units), dim3(kWG), 0, st, g->d_off, g->d_adj, g->d_edge_units,
-                       g->d_cnt, plus, g->d_sum);
-    ACC_HIP_TRY(hipGetLastError());
-  }
+  ACC_HIP_TRY(hipGetLastError());
+  ACC_TRY(launch_units(k_edge_wave<true>, (p.n_units[kEdgeWave] + kWaves - 1) / kWaves, st, g->d_off, g->d_adj,
+                       g->d_rev, g->n, p.d_units[kEdgeWave], p.n_units[kEdgeWave], g->d_cnt, plus, g->d_sum));
+  // d_u <= 2048: one window of 2048 ids; the rest: windows of 8192 ids (38 KB
+  // of LDS: four 512-thread workgroups per CU)
+  ACC_TRY(launch_units(k_merge_win<11, kU>, p.n_units[kEdgeHash12], st, g->d_off, g->d_adj, g->d_rev, g->n,
+                       p.d_units[kEdgeHash12], g->d_cnt, plus, g->d_sum));
+  for (int c : {kEdgeHash14, kEdgeHash15, kEdgeHub})
+    ACC_TRY(launch_units(k_merge_win<13, kMergeU>, p.n_units[c], st, g->d_off, g->d_adj, g->d_rev, g->n, p.d_units[c],
+                         g->d_cnt, plus, g->d_sum));
   return ACC_OK;
 }
 

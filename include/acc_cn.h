@@ -113,6 +113,11 @@ int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap)
 #define ACC_PARAM_CHUNK 4      /* upper slots per workgroup unit (64..512), all paths */
 #define ACC_PARAM_AUTO_PATH 5  /* the path ACC_PATH_AUTO runs */
 #define ACC_PARAM_PERSIST 6    /* forward counting grid: 0 one workgroup per task, k>0 at most k per CU (persistent) */
+#define ACC_PARAM_BMP_HASH_MAX 7 /* bitmap path: sources with d_u at most this (0..16384, default 256) keep
+                                    N(u) in an LDS hash table, larger ones in an HBM bitmap; both behind an
+                                    LDS index of N(u) */
+#define ACC_PARAM_BMP_BATCH 8  /* bitmap path: hub bitmaps built, probed and cleared per batch (1..16384;
+                                  the pool of one batch holds at most 8 GiB) */
 int acc_graph_set_param(acc_graph *g, int param, int64_t value);
 int acc_graph_get_param(const acc_graph *g, int param, int64_t *value);
 

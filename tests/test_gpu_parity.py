@@ -381,3 +381,26 @@ def test_full_size_chunglu23(ctx):
         assert np.array_equal(res["cluster_id"][core], want["cluster_id"][core]), eps
         for k in ("core_num", "cluster_num", "hub_num", "outlier_num"):
             assert res[k] == want[k], (eps, k)
+
+
+@pytest.mark.parametrize("hash_max", [0, 100, 256, 2048, 8192, 16384])
+def test_bitmap_hash_and_hub_classes(ctx, hash_max):
+    """The bitmap path with every split between LDS hash tables (2^12, 2^14,
+    2^15 keys) and HBM hub bitmaps: hash_max 0 sends every source to the hub
+    bitmaps (ER-16384 then needs four batches of 4096 hubs), 100 splits small
+    sources across both; counts bit-exact against the oracle, and the merge
+    path's windows of 2^11 and 2^14 ids alongside."""
+    graphs = [G.gen_er(16384, 16, seed=20190101), G.gen_rmat(17, 16, seed=7),
+              G.gen_chung_lu(1 << 16, 2.1, 32, 40000, seed=8), wide_span_graph(150_000, 1000, 2000, 300, seed=75)]
+    for g in graphs:
+        print(f"[bitmap classes] {g.name} n={g.n} m={g.m} max_deg={int(g.degrees().max())} hash_max={hash_max}",
+              flush=True)
+        ref = oracle.cn_bitmap(g)
+        t_ref = int((ref.astype(np.int64) - 2).sum()) // 6
+        with ctx.upload(g) as rg:
+            rg.set_param("bmp_hash_max", hash_max)
+            for path in ("bitmap", "merge"):
+                cnt, t = rg.count_all_edges(path)
+                bad = np.nonzero(cnt != ref)[0]
+                assert bad.size == 0, f"{path}: {bad.size} slots differ, first {bad[:5]}"
+                assert t == t_ref
