@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "acc_cn.h"
+#include "fwd_plan.h"
 
 // Sets the thread's last error and returns code.
 int acc_set_error(int code, const std::string &msg);
@@ -54,27 +55,7 @@ namespace accnoalloc {
 int bind_scratch(int device, unsigned long long **addr);
 }  // namespace accnoalloc
 
-namespace accfwd {
-constexpr int kWG = 512;                              // threads per workgroup (8 waves)
-constexpr int kWaves = kWG / 64;
-constexpr int kTableBits = 12;                        // workgroup LDS hash table of 4096 keys
-constexpr uint32_t kLdsCap = 1u << (kTableBits - 1);  // |N<(b)| up to 2048 stays in LDS
-constexpr uint32_t kChunk = 512;                      // upper slots per workgroup unit
-constexpr uint32_t kBitmapIds = 32u << kTableBits;    // b - min N(b) <= 2^17: N<(b) as an LDS bitmap of [min N(b), b)
-constexpr uint32_t kBitmapMaxLow = 65535;             // word prefixes are 16-bit
-constexpr int kWaveTableBits = 9;                     // wave-private table of 512 keys
-constexpr uint32_t kWaveCap = 1u << (kWaveTableBits - 1);  // |N<(b)| up to 256 in a wave unit
-constexpr uint32_t kWaveSlots = 64;                   // upper slots of a wave unit
-constexpr int kBigWG = 256;
-}  // namespace accfwd
 
-// One forward-path work unit: source b, its upper slots [j0, j1), nlow = |N<(b)|.
-struct FwdUnit {
-  int32_t b;
-  uint32_t j0;
-  uint32_t j1;
-  uint32_t nlow;
-};
 
 // Per-edge plans (bitmap and merge paths): chunks of at most kChunk upper
 // slots of every source, by d_u.  Wave units: a source with d_u <= 256 and at

@@ -1,0 +1,242 @@
+"""The forward path's work plan on the CPU: the planner of
+acctricnt_amd/csrc/fwd_plan.h (the header the HIP library compiles), built
+here with g++, and an emulation of the wave-pack kernel's index arithmetic
+(fwd_pack_unit in acctricnt_amd/csrc/k_forward.hip) over its output.
+
+For every pack the emulation recomputes what each lane of the kernel
+computes -- the sources' upper-slot, lower-entry and hash-table prefixes, each
+probe's upper slot, list start and length, each lower entry's source and
+table -- and checks that every index stays inside its array, that every
+probe that reads elements has a hash table to look them up in (a source with
+upper slots and an empty N<(b) has none: the kernel gives its probes length 0,
+or the lookup would never meet an empty entry), that the packs respect the
+limits the kernel's LDS layout assumes (64 sources, 64 upper slots, 256
+lower entries, 512 table entries), and that the hits found through this
+arithmetic equal the triangles counted from the definition.  Workgroup units
+must tile each remaining source's upper slots.  No GPU is used."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import bruteforce as BF
+from acctricnt_amd import graph as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "acctricnt_amd", "csrc")
+
+WAVE_CAP, WAVE_TABLE, WAVE_SLOTS = 256, 512, 64  # accfwd::kWaveCap, 1 << kWaveTableBits, kWaveSlots
+LDS_CAP, BITMAP_IDS, BITMAP_MAX_LOW = 2048, 1 << 17, 65535
+
+
+class Unit(C.Structure):
+    _fields_ = [("b", C.c_int32), ("j0", C.c_uint32), ("j1", C.c_uint32), ("nlow", C.c_uint32)]
+
+
+@pytest.fixture(scope="module")
+def planner(tmp_path_factory):
+    out = tmp_path_factory.mktemp("plan") / "libplan.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-pthread", "-I" + CSRC,
+                    os.path.join(ROOT, "tests", "plan", "plan_driver.cpp"), "-o", str(out)],
+                   check=True, capture_output=True, text=True)
+    lib = C.CDLL(str(out))
+    u32p, i32p, up = C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.POINTER(Unit)
+    lib.plan_forward.argtypes = [C.c_uint32, u32p, u32p, i32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.c_uint32, up, u32p, up, u32p, up, u32p]
+    lib.plan_forward.restype = C.c_int
+    return lib
+
+
+def vertex_meta(g):
+    """hs[v]: first slot of N(v) above v; lo[v] = min(N(v) and v) (k_vertex_meta)."""
+    off = g.node_off.astype(np.int64)
+    src = g.sources()
+    low = np.bincount(src[g.edge_dst < src], minlength=g.n)
+    hs = (off[:-1] + low).astype(np.uint32)
+    lo = np.arange(g.n, dtype=np.int32)
+    has = np.diff(off) > 0
+    lo[has] = np.minimum(g.edge_dst[off[:-1][has]], lo[has])
+    return hs, lo
+
+
+def plan(lib, g, wave_slots=64, wave_low=256, chunk=512, device_waves=8192):
+    hs, lo = vertex_meta(g)
+    cap = g.m + g.n + 1
+    bufs = [(Unit * cap)() for _ in range(3)]
+    counts = [C.c_uint32() for _ in range(3)]
+    off = np.ascontiguousarray(g.node_off, np.uint32)
+    rc = lib.plan_forward(g.n, off.ctypes.data_as(C.POINTER(C.c_uint32)), hs.ctypes.data_as(C.POINTER(C.c_uint32)),
+                          lo.ctypes.data_as(C.POINTER(C.c_int32)), wave_slots, wave_low, chunk, device_waves, cap,
+                          bufs[0], C.byref(counts[0]), bufs[1], C.byref(counts[1]), bufs[2], C.byref(counts[2]))
+    assert rc == 0
+    return [[(u.b, u.j0, u.j1, u.nlow) for u in bufs[i][:counts[i].value]] for i in range(3)], hs, lo
+
+
+def lane_search(X, e):
+    """Largest k in [0, 64) with X[k] <= e (k_forward.hip lane_search)."""
+    k = 0
+    for st in (32, 16, 8, 4, 2, 1):
+        if X[k + st] <= e:
+            k += st
+    return k
+
+
+def excl_scan(v):
+    return np.concatenate([[0], np.cumsum(v)[:-1]]).astype(np.int64)
+
+
+def pack_bits(nl):
+    bits = 4
+    while (1 << bits) < 2 * nl:
+        bits += 1
+    return bits
+
+
+def emulate_pack(g, hs, rev, pack, guard=True):
+    """One wave pack as fwd_pack_unit computes it; returns the triangles found
+    (hits) and raises AssertionError on any index outside its range."""
+    off, adj = g.node_off.astype(np.int64), g.edge_dst
+    b0, b1, nup_tot, nl_tot = pack
+    nv = b1 - b0
+    assert 1 <= nv <= 64 and 1 <= nup_tot <= WAVE_SLOTS and nl_tot <= WAVE_CAP
+    o0, h, nupv, nlv, tsz, bits = (np.zeros(64, np.int64) for _ in range(6))
+    for lane in range(nv):
+        v = b0 + lane
+        o0[lane], h[lane] = off[v], hs[v]
+        nupv[lane] = off[v + 1] - hs[v]
+        if nupv[lane]:
+            nlv[lane] = hs[v] - off[v]
+            if nlv[lane]:
+                bits[lane] = pack_bits(nlv[lane])
+                tsz[lane] = 1 << bits[lane]
+    Q, NL, TB = excl_scan(nupv), excl_scan(nlv), excl_scan(tsz)
+    tbl_tot = TB[63] + tsz[63]
+    assert nupv.sum() == nup_tot and nlv.sum() == nl_tot, "pack totals differ from its sources"
+    assert tbl_tot <= WAVE_TABLE, f"hash tables of {tbl_tot} entries exceed the wave's {WAVE_TABLE}"
+    # lower entries: four per lane, each into its source's table
+    tables = {}
+    for r in range(4):
+        for lane in range(64):
+            e = lane + 64 * r
+            k = lane_search(NL, e)
+            if e >= nl_tot:
+                continue
+            lpos = e - NL[k]
+            assert 0 <= lpos < nlv[k] and bits[k] > 0
+            a = adj[o0[k] + lpos]
+            assert a < b0 + k, "a lower entry that is not below its source"
+            assert TB[k] + tsz[k] <= tbl_tot
+            tables.setdefault(k, set()).add(int(a))
+    for k, t in tables.items():
+        assert len(t) == nlv[k] and 2 * len(t) <= tsz[k], "a table without a free entry never ends a lookup"
+    # probes: one per lane, laid end to end
+    hits = 0
+    for lane in range(nup_tot):
+        pk = lane_search(Q, lane)
+        j = h[pk] + (lane - Q[pk])
+        v = b0 + pk
+        assert hs[v] <= j < off[v + 1], "a probe slot outside its source's upper slots"
+        c = adj[j]
+        p = rev[j]
+        cb = off[c]
+        if guard:
+            ln = p - cb if (p != 0xFFFFFFFF and bits[pk] > 0) else 0
+        else:
+            ln = p - cb if p != 0xFFFFFFFF else 0
+        if ln == 0:
+            continue
+        assert bits[pk] > 0, "a probe with elements into a source without a table (the lookup never ends)"
+        assert off[c] <= cb + ln - 1 < off[c + 1] and adj[cb + ln - 1] < v, "a probe element at or above b"
+        hits += sum(1 for a in adj[cb:cb + ln] if int(a) in tables.get(pk, ()))
+    return hits
+
+
+def middle_triangles(g, v):
+    """Triangles whose middle vertex is v, from the definition."""
+    nb = g.edge_dst[g.node_off[v]:g.node_off[v + 1]]
+    lower = set(int(x) for x in nb if x < v)
+    t = 0
+    for c in nb[nb > v]:
+        nc = g.edge_dst[g.node_off[c]:g.node_off[c + 1]]
+        t += sum(1 for a in nc if a < v and int(a) in lower)
+    return t
+
+
+def empty_low_source_graph():
+    """Source 1 has no lower neighbor but an upper neighbor 5 whose list holds
+    0 < 1: its probe of N(5) is non-empty while N<(1) is empty."""
+    s = [0, 1, 0, 2, 3, 2, 4, 6]
+    d = [5, 5, 2, 5, 5, 3, 6, 7]
+    return G.from_edges(8, s, d, name="empty-low")
+
+
+GRAPHS = {
+    "rmat11": lambda: G.gen_rmat(11, 16, seed=3),
+    "rmat11_raw": lambda: G.gen_rmat(11, 8, seed=4, relabel=False),
+    "er3000": lambda: G.gen_er(3000, 12, seed=5),
+    "chunglu4000": lambda: G.gen_chung_lu(4000, 2.1, 10, 600, seed=6),
+    "empty_low": empty_low_source_graph,
+    "wheel9": lambda: BF.wheel(9),
+    "K300": lambda: BF.complete(300),
+}
+
+
+@pytest.mark.parametrize("params", [dict(), dict(wave_slots=16, wave_low=64, chunk=128, device_waves=64),
+                                    dict(wave_slots=0), dict(wave_low=0, chunk=64),
+                                    dict(device_waves=10 ** 7)], ids=["default", "small_bins", "no_packs",
+                                                                      "empty_low_only", "one_source_packs"])
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_plan_bounds_and_pack_arithmetic(planner, name, params):
+    g = GRAPHS[name]()
+    (waves, blocks, bigs), hs, lo = plan(planner, g, **params)
+    rev = g.reverse_slots()
+    off = g.node_off.astype(np.int64)
+    covered = np.zeros(g.n, np.int64)  # upper slots planned per source
+    for pack in waves:
+        b0, b1, nup_tot, nl_tot = pack
+        for v in range(b0, b1):
+            nup = off[v + 1] - hs[v]
+            nl = hs[v] - off[v]
+            if nup:
+                assert nup <= params.get("wave_slots", 64) and nl <= params.get("wave_low", 256)
+            covered[v] += nup
+        hits = emulate_pack(g, hs, rev, pack)
+        assert hits == sum(middle_triangles(g, v) for v in range(b0, b1)), pack
+    for unit, lds in [(u, True) for u in blocks] + [(u, False) for u in bigs]:
+        b, j0, j1, nlow = unit
+        assert hs[b] <= j0 < j1 <= off[b + 1] and j1 - j0 <= params.get("chunk", 512)
+        assert nlow == hs[b] - off[b]
+        fits = nlow <= LDS_CAP or (b - lo[b] <= BITMAP_IDS and nlow <= BITMAP_MAX_LOW)
+        assert fits == lds
+        covered[b] += j1 - j0
+    assert np.array_equal(covered, off[1:] - hs.astype(np.int64)), "upper slots not planned exactly once"
+
+
+def test_empty_low_source_needs_the_guard(planner):
+    """Without the kernel's rule (no table: probe length 0), source 1's probe
+    of N(5) would look up 0 in a table that does not exist; the emulation
+    reports it.  The kernel source still carries the rule."""
+    g = empty_low_source_graph()
+    (waves, _, _), hs, _ = plan(planner, g)
+    rev = g.reverse_slots()
+    pack = next(p for p in waves if p[0] <= 1 < p[1])
+    emulate_pack(g, hs, rev, pack)
+    with pytest.raises(AssertionError, match="without a table"):
+        emulate_pack(g, hs, rev, pack, guard=False)
+    src = open(os.path.join(CSRC, "k_forward.hip")).read()
+    assert "len = p != ~0u && p_bits > 0 ? p - cb : 0;" in src
+
+
+def test_pack_limits_are_checked(planner):
+    """Packs beyond the kernel's LDS layout are reported: 257 lower entries,
+    or tables of more than 512 entries."""
+    g = BF.complete(300)
+    hs, _ = vertex_meta(g)
+    rev = g.reverse_slots()
+    with pytest.raises(AssertionError):
+        emulate_pack(g, hs, rev, (257, 258, 42, 257))
+    # sources 2..9 of K300: 2..9 lower entries each, 16-entry tables; claim fewer than they hold
+    with pytest.raises(AssertionError, match="totals"):
+        emulate_pack(g, hs, rev, (2, 10, 64, 10))
