@@ -1,7 +1,9 @@
 """The forward path's work plan on the CPU: the planner of
 acctricnt_amd/csrc/fwd_plan.h (the header the HIP library compiles), built
 here with g++, and an emulation of the wave-pack kernel's index arithmetic
-(fwd_pack_unit in acctricnt_amd/csrc/k_forward.hip) over its output.
+(fwd_pack_unit in acctricnt_amd/csrc/k_forward.hip) over its output, and of
+the windowed probe searches of both (block_probe_flat's over s_P,
+lane_window_search's over the lanes of a wave pack).
 
 For every pack the emulation recomputes what each lane of the kernel
 computes -- the sources' upper-slot, lower-entry and hash-table prefixes, each
@@ -240,3 +242,149 @@ def test_pack_limits_are_checked(planner):
     # sources 2..9 of K300: 2..9 lower entries each, 16-entry tables; claim fewer than they hold
     with pytest.raises(AssertionError, match="totals"):
         emulate_pack(g, hs, rev, (2, 10, 64, 10))
+
+
+def window_search(P, e0, U, T, first_step=lambda w, hp: w + 1 - hp):
+    """k(e) of every element of the round at e0 as block_probe_flat computes
+    it (a two-level ballot search for the round's first and last probes,
+    then a binary search of that window per element); P is s_P (512 entries,
+    T beyond the chunk's probes).  Returns (k per element row, klo, khi) and
+    raises AssertionError on a read outside [0, 512) or outside the window."""
+    assert len(P) == 512 and e0 < T
+    e1 = min(e0 + 64 * U - 1, T - 1)
+    coarse = P[0:512:8]
+    glo = 8 * (int(np.count_nonzero(coarse <= e0)) - 1)
+    ghi = 8 * (int(np.count_nonzero(coarse <= e1)) - 1)
+    assert 0 <= glo <= ghi <= 504
+    klo = glo + sum(1 for j in range(1, 8) if P[glo + j] <= e0)
+    khi = ghi + sum(1 for j in range(1, 8) if P[ghi + j] <= e1)
+    w = khi - klo
+    ks = np.full((U, 64), klo, np.int64)
+    for q in range(U):
+        for lane in range(64):
+            e = e0 + 64 * q + lane
+            k = klo
+            if w:
+                hp = 1 << (w.bit_length() - 1)
+                d = klo + first_step(w, hp)
+                assert klo <= d <= khi, "a first step outside the window"
+                if P[d] <= e:
+                    k = d
+                st = hp >> 1
+                while st:
+                    assert klo <= k + st <= khi, "a read outside the window"
+                    if P[k + st] <= e:
+                        k += st
+                    st >>= 1
+            ks[q, lane] = k
+    return ks, klo, khi
+
+
+def chunk_prefix(lengths):
+    """s_P of a chunk: exclusive prefix of the probe lengths, T beyond them."""
+    P = np.zeros(512, np.int64)
+    ex = np.concatenate([[0], np.cumsum(lengths)])
+    P[:len(lengths)] = ex[:-1]
+    P[len(lengths):] = ex[-1]
+    return P, int(ex[-1])
+
+
+def chunk_cases():
+    rng = np.random.default_rng(11)
+    yield "one_long", [5000]
+    yield "all_ones", [1] * 512
+    yield "zeros_between", [0, 3, 0, 0, 70, 0, 1, 0, 0, 0, 300, 0]
+    yield "leading_zeros", [0] * 40 + [9] * 30 + [0] * 20
+    yield "full_chunk_skew", list((rng.pareto(1.2, 512) * 8).astype(int))
+    yield "sparse", list(rng.integers(0, 3, 512))
+    yield "short", [2]
+    yield "mid_sizes", list(rng.integers(0, 700, 37))
+
+
+@pytest.mark.parametrize("U", [1, 2, 4])
+@pytest.mark.parametrize("case", [c[0] for c in chunk_cases()])
+def test_block_window_search(case, U):
+    """Every element's probe equals the largest k with s_P[k] <= e, and lanes
+    past the chunk's last element get the window's last probe (so a round's
+    probe indices never decrease across lanes)."""
+    lengths = dict(chunk_cases())[case]
+    P, T = chunk_prefix(lengths)
+    for e0 in range(0, T, 64 * U):
+        ks, klo, khi = window_search(P, e0, U, T)
+        for q in range(U):
+            e = e0 + 64 * q + np.arange(64)
+            want = np.searchsorted(P, e, side="right") - 1
+            valid = e < T
+            assert np.array_equal(ks[q][valid], want[valid]), (case, e0, q)
+            assert np.all(ks[q][~valid] == khi)
+            assert np.all(np.diff(ks[q]) >= 0)
+            k = ks[q][valid]
+            assert np.all(P[k] <= e[valid]) and np.all(P[np.minimum(k + 1, 511)][k < 511] > e[valid][k < 511])
+
+
+def test_block_window_search_catches_a_short_first_step():
+    """A first step one entry short leaves the window's last probe out of
+    reach: the emulation reports wrong probes."""
+    P, T = chunk_prefix([0, 3, 0, 0, 70, 0, 1, 0, 0, 0, 300, 0])
+    bad = 0
+    for e0 in range(0, T, 64):
+        ks, _, _ = window_search(P, e0, 1, T, first_step=lambda w, hp: w - hp if w > hp else 1)
+        e = e0 + np.arange(64)
+        valid = e < T
+        bad += int(np.count_nonzero(ks[0][valid] != (np.searchsorted(P, e, side="right") - 1)[valid]))
+    assert bad > 0
+    src = open(os.path.join(CSRC, "k_forward.hip")).read()
+    assert "const uint32_t d = klo + w + 1 - hp;" in src
+
+
+def lane_window(X, e0, U, T):
+    """lane_window_search (k_forward.hip) for the round at e0: X holds the 64
+    lanes' exclusive prefix of the pack's probe lengths (T beyond its last
+    probe).  Returns k per element row; AssertionError on a lane read outside
+    [klo, khi]."""
+    assert len(X) == 64 and e0 < T
+    e1 = min(e0 + 64 * U - 1, T - 1)
+    klo = int(np.count_nonzero(X <= e0)) - 1
+    khi = int(np.count_nonzero(X <= e1)) - 1
+    w = khi - klo
+    ks = np.full((U, 64), klo, np.int64)
+    for q in range(U):
+        for lane in range(64):
+            e = e0 + 64 * q + lane
+            k = klo
+            if w:
+                hp = 1 << (w.bit_length() - 1)
+                d = klo + w + 1 - hp
+                if X[d] <= e:
+                    k = d
+                st = hp >> 1
+                while st:
+                    assert klo <= k + st <= khi, "a lane read outside the window"
+                    if X[k + st] <= e:
+                        k += st
+                    st >>= 1
+            ks[q, lane] = k
+    return ks, khi
+
+
+@pytest.mark.parametrize("U", [1, 2, 4])
+@pytest.mark.parametrize("seed", range(6))
+def test_lane_window_search(seed, U):
+    """The wave pack's per-element probe equals lane_search's (the largest
+    lane k with X_k <= e) for every element, with zero-length probes and lanes
+    past the pack's probes (length 0, X = T) included."""
+    rng = np.random.default_rng(100 + seed)
+    nup = int(rng.integers(1, 65))
+    lengths = np.zeros(64, np.int64)
+    lengths[:nup] = rng.integers(0, [3, 40, 300, 5, 2000, 64][seed], nup)
+    if lengths.sum() == 0:
+        lengths[nup - 1] = 1
+    X = np.concatenate([[0], np.cumsum(lengths)[:-1]])
+    T = int(lengths.sum())
+    for e0 in range(0, T, 64 * U):
+        ks, khi = lane_window(X, e0, U, T)
+        for q in range(U):
+            for lane in range(64):
+                e = e0 + 64 * q + lane
+                want = lane_search(X, e) if e < T else khi
+                assert ks[q, lane] == want, (seed, e0, q, lane)
