@@ -19,6 +19,7 @@
 // (scan-query/scan_xp.cpp:675-857) for every slot.
 
 #include "acc_internal.h"
+#include "flat_window.h"
 
 namespace {
 
@@ -295,55 +296,20 @@ constexpr size_t kCountLdsBytes = kBlockLdsBytes > kWaveLdsBytes ? kBlockLdsByte
 // lengths and s_cb the start of each N(c_k) less its prefix, so that element
 // e of probe k is slot s_cb[k] + e (mod 2^32).  Wave w takes the rounds
 // w, w + kWaves, ... of 64 * U consecutive elements, so the chunk's work is
-// split evenly over the waves whatever the list lengths.  k(e), the largest
-// k with s_P[k] <= e, is found in two stages: per round, the probes klo and
-// khi of the round's first and last element by a two-level ballot search
-// (the 64 lanes test every 8th entry, then 8 lanes per end test the 8
-// entries of its group); per element, a binary search of [klo, khi] only,
-// ceil(log2(khi - klo + 1)) steps, most rounds spanning a few probes.
-// Round indices and both window ends are wave-uniform.
+// split evenly over the waves whatever the list lengths; k(e) comes from a
+// per-round window search (flat_window.h).  Round indices are wave-uniform.
 template <int U, bool kSlots, class Set>
 __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__ adj, int32_t *cnt, const Set &set,
                                                      int32_t *ctr, uint32_t sb, const uint32_t *s_P,
                                                      const uint32_t *s_cb, uint32_t T, int32_t *s_reg, int wave,
                                                      int lane) {
-  static_assert(kChunk == 64 * 8, "the window search below covers 64 groups of 8 entries");
+  static_assert(kChunk == 64 * 8, "the window search covers 64 groups of 8 entries");
   uint32_t found = 0;
-  const bool lo_end = lane < 8;
   for (uint32_t e0 = static_cast<uint32_t>(wave) * 64 * U; e0 < T; e0 += kWaves * 64 * U) {
-    const uint32_t e1 = e0 + 64 * U - 1 < T ? e0 + 64 * U - 1 : T - 1;
-    const uint32_t coarse = s_P[8 * lane];
-    const uint32_t glo = 8 * (__popcll(__ballot(coarse <= e0)) - 1);
-    const uint32_t ghi = 8 * (__popcll(__ballot(coarse <= e1)) - 1);
-    const uint32_t fine = s_P[(lo_end ? glo : ghi) + (lane & 7)];
-    const unsigned long long fb = __ballot(fine <= (lo_end ? e0 : e1));
-    const uint32_t klo = glo + __popcll(fb & 0xFEull);
-    const uint32_t khi = ghi + __popcll(fb & 0xFE00ull);
     int32_t a[U];
     uint32_t idx[U];
     int kq[U];
-#pragma unroll
-    for (int q = 0; q < U; q++) kq[q] = static_cast<int>(klo);
-    const uint32_t w = khi - klo;
-    if (w) {
-      // w + 1 candidates: a first step of w + 1 - hp, then hp / 2, ..., 1
-      // (hp the largest power of two <= w); every read stays in [klo, khi]
-      const uint32_t hp = 1u << (31 - __builtin_clz(w));
-      const uint32_t d = klo + w + 1 - hp;
-      const uint32_t pd = s_P[d];
-#pragma unroll
-      for (int q = 0; q < U; q++)
-        if (pd <= e0 + 64 * q + lane) kq[q] = static_cast<int>(d);
-      // the U searches advance together, one LDS read of each per step
-      for (uint32_t st = hp >> 1; st; st >>= 1) {
-        uint32_t p[U];
-#pragma unroll
-        for (int q = 0; q < U; q++) p[q] = s_P[kq[q] + st];
-#pragma unroll
-        for (int q = 0; q < U; q++)
-          if (p[q] <= e0 + 64 * q + lane) kq[q] += st;
-      }
-    }
+    accflat::lds_search<U>(s_P, e0, accflat::round_last<U>(e0, T), lane, kq);
 #pragma unroll
     for (int q = 0; q < U; q++) {
       const uint32_t e = e0 + 64 * q + lane;
@@ -467,38 +433,6 @@ __device__ __forceinline__ int lane_search(uint32_t X, uint32_t e) {
   return k;
 }
 
-// k(e), the largest lane k with X_k <= e, for the U element rows
-// e = e0 + 64 q + lane of one round, X nondecreasing across the lanes with
-// X_0 = 0 and T = X_63 + the last lane's length: two ballots give the probes
-// klo and khi of the round's first and last element, and each element
-// searches only [klo, khi] (a first step of w + 1 - hp, then hp / 2, ..., 1,
-// for w = khi - klo and hp the largest power of two <= w), so a round inside
-// one probe needs no search at all.  Lanes past T get khi.
-template <int U>
-__device__ __forceinline__ void lane_window_search(uint32_t X, uint32_t e0, uint32_t T, int lane, int (&k)[U]) {
-  const uint32_t e1 = e0 + 64 * U - 1 < T ? e0 + 64 * U - 1 : T - 1;
-  const int klo = static_cast<int>(__popcll(__ballot(X <= e0))) - 1;
-  const int khi = static_cast<int>(__popcll(__ballot(X <= e1))) - 1;
-#pragma unroll
-  for (int q = 0; q < U; q++) k[q] = klo;
-  const uint32_t w = static_cast<uint32_t>(khi - klo);
-  if (!w) return;
-  const uint32_t hp = 1u << (31 - __builtin_clz(w));
-  const int d = klo + static_cast<int>(w + 1 - hp);
-  const uint32_t xd = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(X), d));
-#pragma unroll
-  for (int q = 0; q < U; q++)
-    if (xd <= e0 + 64 * q + lane) k[q] = d;
-  for (int st = static_cast<int>(hp >> 1); st; st >>= 1) {
-    uint32_t v[U];
-#pragma unroll
-    for (int q = 0; q < U; q++) v[q] = static_cast<uint32_t>(__shfl(X, k[q] + st));
-#pragma unroll
-    for (int q = 0; q < U; q++)
-      if (v[q] <= e0 + 64 * q + lane) k[q] += st;
-  }
-}
-
 // Hash table size of one source of a wave pack: the smallest power of two
 // >= 2 |N<(b)|, at least 16 entries.
 __device__ __forceinline__ int pack_bits(uint32_t nl) {
@@ -612,7 +546,7 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
     uint32_t eq[U];
 #pragma unroll
     for (int q = 0; q < U; q++) eq[q] = e0 + 64 * q + lane;
-    lane_window_search<U>(P, e0, T, lane, kq);
+    accflat::lane_window<U>(P, e0, accflat::round_last<U>(e0, T), lane, kq);
 #pragma unroll
     for (int q = 0; q < U; q++) {
       idx[q] = static_cast<uint32_t>(__shfl(cbP, kq[q])) + eq[q];

@@ -29,7 +29,7 @@
 // one source u and at most kChunk of its upper slots (u, v); the lists N(v)
 // (or their slices) are laid end to end, element e of the unit's range
 // belongs to probe k(e) = max{k : P_k <= e} over the exclusive prefix P of
-// the lengths (a 9-step search in LDS), each lane takes one element per
+// the lengths (a window search, flat_window.h), each lane takes one element per
 // step, so no lane idles on a short list, and per-probe hit counts go to
 // LDS counters (one popcount when a 64-element round lies in one probe).
 // Each unit then writes 2 + |N(u) ∩ N(v)| (or the raw count) into the upper
@@ -38,6 +38,7 @@
 // triangle sum.
 
 #include "acc_internal.h"
+#include "flat_window.h"
 
 namespace {
 
@@ -124,53 +125,20 @@ __device__ __forceinline__ uint32_t flatten(UnitLds &L, uint32_t cb, uint32_t le
 
 // Probes the T flattened elements against set.  Wave w takes the contiguous
 // range [w R, (w + 1) R) of the elements (R a multiple of 64, the ranges
-// equal but for the last), in steps of kU * 64.  Each step first advances a
-// wave-uniform probe index kc to the probe of its first element (one LDS
-// read while that probe goes on, a binary search of P over [kc, 512)
-// otherwise); when the step ends inside probe kc, every lane takes kc, else
-// each lane finds its element's probe by a 9-step search of P.  Step indices
-// and kc are wave-uniform.
+// equal but for the last), in steps of U * 64; each step's elements find
+// their probes by the window search of flat_window.h.  Step indices are
+// wave-uniform.
 template <int U = kU, class Set>
 __device__ __forceinline__ void probe_flat(const int32_t *__restrict__ adj, const Set &set, UnitLds &L, uint32_t T) {
-  static_assert(kChunk == 512, "the search below covers 512 entries");
+  static_assert(kChunk == 64 * 8, "the window search covers 64 groups of 8 entries");
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t R = ((T + kWaves - 1) / kWaves + 63) & ~63u;
   const uint32_t lo = min(T, static_cast<uint32_t>(wave) * R), hi = min(T, lo + R);
-  int kc = 0;
   for (uint32_t e0 = lo; e0 < hi; e0 += 64 * U) {
-    if (kc < static_cast<int>(kChunk) - 1 && L.P[kc + 1] <= e0) {
-      int lo_k = kc + 1, hi_k = static_cast<int>(kChunk) - 1;  // the largest k with P[k] <= e0 lies in [lo_k, hi_k]
-      while (lo_k < hi_k) {
-        const int mid = (lo_k + hi_k + 1) >> 1;
-        if (L.P[mid] <= e0)
-          lo_k = mid;
-        else
-          hi_k = mid - 1;
-      }
-      kc = lo_k;
-    }
-    kc = __builtin_amdgcn_readfirstlane(kc);
-    const uint32_t last = min(e0 + 64 * U, hi) - 1;
-    const bool one = kc == static_cast<int>(kChunk) - 1 || L.P[kc + 1] > last;
     int32_t a[U];
     int kq[U];
-#pragma unroll
-    for (int q = 0; q < U; q++) kq[q] = kc;
-    if (!one) {
-      // the U searches advance together, one LDS read of each per step
-#pragma unroll
-      for (int q = 0; q < U; q++) kq[q] = 0;
-#pragma unroll
-      for (int st = 256; st; st >>= 1) {
-        uint32_t p[U];
-#pragma unroll
-        for (int q = 0; q < U; q++) p[q] = L.P[kq[q] + st];
-#pragma unroll
-        for (int q = 0; q < U; q++)
-          if (p[q] <= e0 + 64 * q + lane) kq[q] += st;
-      }
-    }
+    accflat::lds_search<U>(L.P, e0, accflat::round_last<U>(e0, hi), lane, kq);
 #pragma unroll
     for (int q = 0; q < U; q++) {
       const uint32_t e = e0 + 64 * q + lane;
@@ -322,7 +290,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // Wave units: one source u (d_u <= kEdgeWaveDeg, at most 64 upper slots) per
 // wave, eight per workgroup, with no workgroup barrier: lane t holds probe t
 // (upper slot j0 + t), the wave lays the probes end to end and finds each
-// element's probe by a 6-step search over the lanes' prefix.  N(u) is a
+// element's probe by a window search over the lanes' prefix.  N(u) is a
 // wave-private LDS hash table (bitmap path) or sorted run (merge path).
 template <bool MERGE>
 __global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
@@ -369,19 +337,7 @@ __global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ 
     for (uint32_t e0 = 0; e0 < T; e0 += 64 * kU) {
       int32_t a[kU];
       int kq[kU];
-#pragma unroll
-      for (int q = 0; q < kU; q++) kq[q] = 0;
-#pragma unroll
-      for (int st = 32; st; st >>= 1) {  // the kU lane searches advance together
-        uint32_t v[kU];
-#pragma unroll
-        for (int q = 0; q < kU; q++) v[q] = static_cast<uint32_t>(__shfl(P, kq[q] + st));
-#pragma unroll
-        for (int q = 0; q < kU; q++) {
-          const uint32_t e = e0 + 64 * q + lane;
-          if (v[q] <= (e < T ? e : T - 1)) kq[q] += st;
-        }
-      }
+      accflat::lane_window<kU>(P, e0, accflat::round_last<kU>(e0, T), lane, kq);
 #pragma unroll
       for (int q = 0; q < kU; q++) {
         const uint32_t e = e0 + 64 * q + lane;

@@ -31,8 +31,9 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # params: the forward-path parameters acc_tune picks for the workload
-    # (chunk 128 against the default 512: 23.9 against 24.6 ms a pass)
-    "rmat22": dict(kind="rmat", scale=22, edge_factor=16, seed=1, params={"chunk": 128}),
+    # (chunk 256 against 128 and the default 512: 21.9 against 23.2 and
+    # 22.0 ms a pass, unroll 4)
+    "rmat22": dict(kind="rmat", scale=22, edge_factor=16, seed=1, params={"chunk": 256}),
     "rmat20": dict(kind="rmat", scale=20, edge_factor=16, seed=1),
     "chunglu23": dict(kind="chung_lu", n=1 << 23, gamma=2.1, avg_deg=32, max_deg=1 << 20, seed=1),
     "er16384": dict(kind="er", n=16384, avg_deg=16, seed=20190101),

@@ -302,8 +302,9 @@ def rmat22():
     return g, ref, t
 
 
-@pytest.mark.parametrize("path,params", [("forward", {}), ("forward", {"chunk": 128}), ("bitmap", {}),
-                                         ("merge", {})], ids=["forward", "forward_chunk128", "bitmap", "merge"])
+@pytest.mark.parametrize("path,params", [("forward", {}), ("forward", {"chunk": 128}),
+                                         ("forward", {"chunk": 256, "fwd_unroll": 2}), ("bitmap", {}), ("merge", {})],
+                         ids=["forward", "forward_chunk128", "forward_chunk256_u2", "bitmap", "merge"])
 def test_full_size_rmat22(ctx, rmat22, path, params):
     """R-MAT scale 22 / ef 16, every one of the 128.3M slots bit-exact against
     the oracle's counts on each path, T equal to the oracle's triangle count,

@@ -2,8 +2,9 @@
 acctricnt_amd/csrc/fwd_plan.h (the header the HIP library compiles), built
 here with g++, and an emulation of the wave-pack kernel's index arithmetic
 (fwd_pack_unit in acctricnt_amd/csrc/k_forward.hip) over its output, and of
-the windowed probe searches of both (block_probe_flat's over s_P,
-lane_window_search's over the lanes of a wave pack).
+the windowed probe searches of acctricnt_amd/csrc/flat_window.h that the
+forward and per-edge kernels share (lds_search over a 512-entry prefix in
+LDS, lane_window over the lanes of a wave).
 
 For every pack the emulation recomputes what each lane of the kernel
 computes -- the sources' upper-slot, lower-entry and hash-table prefixes, each
@@ -245,8 +246,8 @@ def test_pack_limits_are_checked(planner):
 
 
 def window_search(P, e0, U, T, first_step=lambda w, hp: w + 1 - hp):
-    """k(e) of every element of the round at e0 as block_probe_flat computes
-    it (a two-level ballot search for the round's first and last probes,
+    """k(e) of every element of the round at e0 as lds_search (flat_window.h)
+    computes it (a two-level ballot search for the round's first and last probes,
     then a binary search of that window per element); P is s_P (512 entries,
     T beyond the chunk's probes).  Returns (k per element row, klo, khi) and
     raises AssertionError on a read outside [0, 512) or outside the window."""
@@ -333,12 +334,12 @@ def test_block_window_search_catches_a_short_first_step():
         valid = e < T
         bad += int(np.count_nonzero(ks[0][valid] != (np.searchsorted(P, e, side="right") - 1)[valid]))
     assert bad > 0
-    src = open(os.path.join(CSRC, "k_forward.hip")).read()
+    src = open(os.path.join(CSRC, "flat_window.h")).read()
     assert "const uint32_t d = klo + w + 1 - hp;" in src
 
 
 def lane_window(X, e0, U, T):
-    """lane_window_search (k_forward.hip) for the round at e0: X holds the 64
+    """lane_window (flat_window.h) for the round at e0: X holds the 64
     lanes' exclusive prefix of the pack's probe lengths (T beyond its last
     probe).  Returns k per element row; AssertionError on a lane read outside
     [klo, khi]."""
