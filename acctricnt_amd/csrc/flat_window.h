@@ -88,6 +88,34 @@ __device__ __forceinline__ void lane_window(uint32_t X, uint32_t e0, uint32_t e1
   }
 }
 
+// Adds one row's hits to the per-probe counters ctr[k] in LDS.  The probe
+// index k is nondecreasing over the lanes (the window search guarantees it,
+// lanes past the end included), so the lanes fall into segments of equal k:
+// the first lane of each segment adds the popcount of its segment's hits --
+// one LDS atomic per probe of the row, never two lanes on one address (a
+// per-hit atomic would serialise all the hits of a probe on one address).
+__device__ __forceinline__ void add_row_hits(int32_t *ctr, int k, bool hit, int lane) {
+  const unsigned long long hits = __ballot(hit);
+  if (!hits) return;
+  typedef __attribute__((address_space(3))) int32_t lds_i32;
+  const int k0 = __builtin_amdgcn_readfirstlane(k);
+  if (k0 == __builtin_amdgcn_readlane(k, 63)) {  // one probe: one add
+    if (lane == 0)
+      __hip_atomic_fetch_add((lds_i32 *)&ctr[k0], static_cast<int>(__popcll(hits)), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    return;
+  }
+  const bool head = lane == 0 || k != __shfl_up(k, 1, 64);
+  const unsigned long long heads = __ballot(head);
+  if (head) {
+    const unsigned long long above = heads & ~((2ull << lane) - 1);  // segment heads after this lane
+    const unsigned long long seg = (above ? (above & (0ull - above)) - 1 : ~0ull) & ~((1ull << lane) - 1);
+    const int c = static_cast<int>(__popcll(hits & seg));
+    if (c)
+      __hip_atomic_fetch_add((lds_i32 *)&ctr[k], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+}
+
 // The round's last element below `end` (end > e0).
 template <int U>
 __device__ __forceinline__ uint32_t round_last(uint32_t e0, uint32_t end) {

@@ -264,7 +264,8 @@ __device__ __forceinline__ void count_triangle(int32_t *cnt, int32_t *ctr, uint3
 
 // Adds the round's hits to the (b,c_k) counters reg[k] (LDS): one add of the
 // ballot's popcount when the whole round lies in one probe, else one LDS
-// atomic per hit.
+// atomic per hit.  (A per-segment add, one per probe of the round, measured
+// no faster here: the (c,a) updates, not these counters, bound the pass.)
 __device__ __forceinline__ void count_round(int32_t *reg, int k, bool hit, int lane) {
   const unsigned long long hits = __ballot(hit);
   if (!hits) return;
@@ -304,35 +305,71 @@ __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__
                                                      const uint32_t *s_cb, uint32_t T, int32_t *s_reg, int wave,
                                                      int lane) {
   static_assert(kChunk == 64 * 8, "the window search covers 64 groups of 8 entries");
+  constexpr uint32_t kStride = kWaves * 64 * U;
   uint32_t found = 0;
-  for (uint32_t e0 = static_cast<uint32_t>(wave) * 64 * U; e0 < T; e0 += kWaves * 64 * U) {
-    int32_t a[U];
-    uint32_t idx[U];
-    int kq[U];
-    accflat::lds_search<U>(s_P, e0, accflat::round_last<U>(e0, T), lane, kq);
+  uint32_t e0 = static_cast<uint32_t>(wave) * 64 * U;
+  if constexpr (kSlots) {
+    for (; e0 < T; e0 += kStride) {
+      int32_t a[U];
+      uint32_t idx[U];
+      int kq[U];
+      accflat::lds_search<U>(s_P, e0, accflat::round_last<U>(e0, T), lane, kq);
 #pragma unroll
-    for (int q = 0; q < U; q++) {
-      const uint32_t e = e0 + 64 * q + lane;
-      idx[q] = s_cb[kq[q]] + e;
-      a[q] = e < T ? adj[idx[q]] : -1;
-    }
-#pragma unroll
-    for (int q = 0; q < U; q++) {
-      bool hit = false;
-      if (a[q] >= 0) {
-        const int pos = set.find(a[q]);
-        if (pos >= 0) {
-          if (kSlots) count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
-          hit = true;
-        }
+      for (int q = 0; q < U; q++) {
+        const uint32_t e = e0 + 64 * q + lane;
+        idx[q] = s_cb[kq[q]] + e;
+        a[q] = e < T ? adj[idx[q]] : -1;
       }
-      if (kSlots)
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        bool hit = false;
+        if (a[q] >= 0) {
+          const int pos = set.find(a[q]);
+          if (pos >= 0) {
+            count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
+            hit = true;
+          }
+        }
         count_round(s_reg, kq[q], hit, lane);
-      else
-        found += hit;
+      }
     }
+    return 0;
+  } else {
+    // Triangle-only: the next round's elements are loaded (unconditionally:
+    // the last round re-reads its own, lanes past T read the round's last
+    // element) before this round's lookups, so the loads of one round are
+    // in flight while the previous round is decided.  (In the per-slot pass
+    // the same pipelining was slower: its count updates, issued between the
+    // loads, are waited for with them.)
+    if (e0 >= T) return 0;
+    struct Round {
+      int32_t v[U];  // element id, -1 past T
+    };
+    auto fetch = [&](uint32_t f0, Round &r) {
+      const uint32_t f1 = accflat::round_last<U>(f0, T);
+      int k[U];
+      accflat::lds_search<U>(s_P, f0, f1, lane, k);
+#pragma unroll
+      for (int q = 0; q < U; q++) {
+        const uint32_t e = f0 + 64 * q + lane;
+        const int32_t x = adj[s_cb[k[q]] + (e < T ? e : f1)];  // past T: probe k(f1), element f1
+        r.v[q] = e < T ? x : -1;
+      }
+    };
+    Round cur, nxt;
+    fetch(e0, cur);
+    for (;;) {
+      const uint32_t e1 = e0 + kStride;
+      const bool more = e1 < T;  // wave-uniform
+      fetch(more ? e1 : e0, nxt);
+#pragma unroll
+      for (int q = 0; q < U; q++) found += cur.v[q] >= 0 && set.find(cur.v[q]) >= 0;
+      if (!more) break;
+      e0 = e1;
+      cur = nxt;
+    }
+    return found;
   }
-  return found;
 }
 
 // Workgroup unit: one chunk of at most kChunk upper slots of a source b.

@@ -66,33 +66,8 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane) {
 
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
-__device__ __forceinline__ void lds_add(int32_t *p, int v) {
-  __hip_atomic_fetch_add((lds_i32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
 __device__ __forceinline__ void lds_or(uint32_t *p, uint32_t v) {
   __hip_atomic_fetch_or((lds_u32 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Adds the round's hits to the probe counters.  The probe index k is
-// nondecreasing over the lanes, so the lanes fall into segments of equal k:
-// the first lane of each segment adds the popcount of the segment's hits (one
-// add in all when the whole round lies in one probe).
-__device__ __forceinline__ void count_round(int32_t *ctr, int k, bool hit, int lane) {
-  const unsigned long long hits = __ballot(hit);
-  if (!hits) return;
-  const int k0 = __builtin_amdgcn_readfirstlane(k);
-  if (k0 == __builtin_amdgcn_readlane(k, 63)) {
-    if (lane == 0) lds_add(&ctr[k0], static_cast<int>(__popcll(hits)));
-    return;
-  }
-  const bool head = lane == 0 || k != __shfl_up(k, 1, 64);
-  const unsigned long long heads = __ballot(head);
-  if (head) {
-    const unsigned long long above = heads & ~((2ull << lane) - 1);  // segment heads after this lane
-    const unsigned long long seg = (above ? (above & (0ull - above)) - 1 : ~0ull) & ~((1ull << lane) - 1);
-    const int c = static_cast<int>(__popcll(hits & seg));
-    if (c) lds_add(&ctr[k], c);
-  }
 }
 
 // Probe metadata of a unit in LDS.
@@ -147,7 +122,7 @@ __device__ __forceinline__ void probe_flat(const int32_t *__restrict__ adj, cons
     bool hit[U];
     set.template has_n<U>(a, hit);
 #pragma unroll
-    for (int q = 0; q < U; q++) count_round(L.ctr, kq[q], hit[q], lane);
+    for (int q = 0; q < U; q++) accflat::add_row_hits(L.ctr, kq[q], hit[q], lane);
   }
 }
 
@@ -350,7 +325,7 @@ __global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ 
       else
         LdsHash{t, bits, mask}.has_n<kU>(a, hit);
 #pragma unroll
-      for (int q = 0; q < kU; q++) count_round(ct, kq[q], hit[q], lane);
+      for (int q = 0; q < kU; q++) accflat::add_row_hits(ct, kq[q], hit[q], lane);
     }
     wave_lds_sync();
     if (static_cast<uint32_t>(lane) < nup) {

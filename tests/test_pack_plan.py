@@ -4,7 +4,8 @@ here with g++, and an emulation of the wave-pack kernel's index arithmetic
 (fwd_pack_unit in acctricnt_amd/csrc/k_forward.hip) over its output, and of
 the windowed probe searches of acctricnt_amd/csrc/flat_window.h that the
 forward and per-edge kernels share (lds_search over a 512-entry prefix in
-LDS, lane_window over the lanes of a wave).
+LDS, lane_window over the lanes of a wave) and of its per-probe hit adds
+(add_row_hits).
 
 For every pack the emulation recomputes what each lane of the kernel
 computes -- the sources' upper-slot, lower-entry and hash-table prefixes, each
@@ -389,3 +390,45 @@ def test_lane_window_search(seed, U):
                 e = e0 + 64 * q + lane
                 want = lane_search(X, e) if e < T else khi
                 assert ks[q, lane] == want, (seed, e0, q, lane)
+
+
+def add_row_hits(k, hit):
+    """add_row_hits (flat_window.h) for one row of 64 lanes: the LDS adds it
+    issues, as {probe: amount}, and the number of adds."""
+    k = [int(x) for x in k]
+    hits = sum(1 << l for l in range(64) if hit[l])
+    if not hits:
+        return {}, 0
+    if k[0] == k[63]:
+        return {k[0]: bin(hits).count("1")}, 1
+    heads = sum(1 << l for l in range(64) if l == 0 or k[l] != k[l - 1])
+    adds, n = {}, 0
+    M = (1 << 64) - 1
+    for l in range(64):
+        if not (heads >> l) & 1:
+            continue
+        above = heads & ~(((2 << l) & M) - 1) & M
+        seg = ((above & -above) - 1 if above else M) & ~((1 << l) - 1) & M
+        c = bin(hits & seg).count("1")
+        if c:
+            assert k[l] not in adds, "two adds on one probe"
+            adds[k[l]] = c
+            n += 1
+    return adds, n
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_row_hits_by_segment(seed):
+    """The per-segment popcounts add up to the per-hit counts of every probe,
+    with at most one add per probe of the row."""
+    rng = np.random.default_rng(200 + seed)
+    for _ in range(50):
+        k = np.sort(rng.integers(0, [1, 2, 5, 64][seed % 4] + 1, 64)) + int(rng.integers(0, 400))
+        hit = rng.random(64) < [0.05, 0.3, 0.9][seed % 3]
+        adds, n = add_row_hits(k, hit)
+        want = {}
+        for l in range(64):
+            if hit[l]:
+                want[int(k[l])] = want.get(int(k[l]), 0) + 1
+        assert adds == want
+        assert n == len(want)
