@@ -63,12 +63,13 @@ __global__ __launch_bounds__(256) void k_init(int32_t *__restrict__ cnt, uint64_
 
 // Forward work estimate of every source b: over its upper slots (b, c) the
 // probe length |N(c) ∩ [0, b)| (the reverse slot minus the start of N(c))
-// plus 4 for the slot's fixed cost plus the probe's hits (hits[j], the
+// plus 4 for the slot's fixed cost plus twice the probe's hits (hits[j], the
 // triangles with middle b and top c: the upper slot's value after a
 // counting pass without the combine), and per unit of `chunk` upper slots
-// twice the build of the N<(b) table plus 512.  The weights are a fit of
-// the share kernel times of 8 shares of R-MAT scale 22 (hits and unit
-// overheads weigh about 1.2 and 1.8 elements).  One wave per source.
+// the build of the N<(b) table plus 256.  The weights follow a least-squares
+// fit of the kernel times of 8 shares of R-MAT scale 22 (a hit weighs about
+// 1.8 probe elements, a unit's table build well under its two earlier
+// elements per entry).  One wave per source.
 __global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict__ off, const uint32_t *__restrict__ hs,
                                                      const int32_t *__restrict__ adj,
                                                      const uint32_t *__restrict__ rev, const int32_t *__restrict__ hits,
@@ -80,11 +81,11 @@ __global__ __launch_bounds__(256) void k_source_work(const uint32_t *__restrict_
     unsigned long long w = 0;
     for (uint32_t j = hs[b] + lane; j < off[b + 1]; j += 64) {
       const uint32_t p = rev[j];
-      w += 4 + (p != ~0u ? p - off[adj[j]] : 0) + static_cast<uint32_t>(hits[j]);
+      w += 4 + (p != ~0u ? p - off[adj[j]] : 0) + 2ull * static_cast<uint32_t>(hits[j]);
     }
     w = wave_sum_u64(w);
     const uint32_t nup = off[b + 1] - hs[b], nl = hs[b] - off[b];
-    if (lane == 0) work[b] = w + static_cast<unsigned long long>((nup + chunk - 1) / chunk) * 2ull * (nl + 256u);
+    if (lane == 0) work[b] = w + static_cast<unsigned long long>((nup + chunk - 1) / chunk) * (nl + 256u);
   }
 }
 

@@ -135,8 +135,8 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap);
 /* Restricts the forward path (ACC_PATH_FORWARD passes and the triangle-only
  * pass) to share `index` of `parts`: the sources cut into `parts`
  * contiguous vertex ranges of equal estimated work (an upper slot (b, c)
- * weighs |N(c) ∩ [0, b)| + 4 + the triangles with middle b and top c, a work
- * unit of up to `chunk` upper slots of b weighs 2 (|N<(b)| + 256)), the share
+ * weighs |N(c) ∩ [0, b)| + 4 + twice the triangles with middle b and top c,
+ * a work unit of up to `chunk` upper slots of b weighs |N<(b)| + 256), the share
  * one GPU of a `parts`-GPU run would count.  While counts are one share's,
  * acc_scan_run refuses to cluster on them (ACC_EINVAL) instead of counting
  * the whole graph; acc_triangle_total on the forward path returns the share's

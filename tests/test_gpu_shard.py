@@ -58,9 +58,9 @@ def test_shares_add_up_to_the_whole(ctx, name, parts):
 
 def test_shares_balance_probe_work(ctx):
     """Eight shares of an R-MAT graph: no share's estimated work (per upper
-    slot (b, c) its probe length + 4 + the triangles with middle b and top c,
-    2 (|N<(b)| + 256) per 512-slot unit) exceeds an equal split by more than
-    its heaviest single source."""
+    slot (b, c) its probe length + 4 + twice the triangles with middle b and
+    top c, |N<(b)| + 256 per 512-slot unit) exceeds an equal split by more
+    than its heaviest single source."""
     import scipy.sparse as sp
 
     g = G.gen_rmat(17, 16, seed=95)
@@ -73,11 +73,11 @@ def test_shares_balance_probe_work(ctx):
     A = sp.csr_matrix((np.ones(g.m, np.int64), g.edge_dst, g.node_off.astype(np.int64)), shape=(g.n, g.n))
     hits = np.asarray((sp.tril(A, -1).tocsr() @ A)[src[up], dst[up]]).ravel()
     work = np.zeros(g.n, np.int64)
-    np.add.at(work, src[up], lens + 4 + hits)
+    np.add.at(work, src[up], lens + 4 + 2 * hits)
     hs = off[:-1] + np.bincount(src[~up], minlength=g.n)  # lower slots come first in each list
     nup = off[1:] - hs
     nl = hs - off[:-1]
-    work += (nup + 511) // 512 * 2 * (nl + 256)
+    work += (nup + 511) // 512 * (nl + 256)
     total = int(work.sum())
     with ctx.upload(g) as rg:
         for s in range(8):
