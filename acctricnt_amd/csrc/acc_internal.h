@@ -77,7 +77,8 @@ struct EdgePlan {
   uint32_t *d_pool = nullptr;            // bitmaps of one batch of hubs, zero between passes
   uint32_t pool_hubs = 0;
   uint64_t pool_words = 0;               // words per bitmap: ceil(n / 32)
-  int idx_shift = 0;                     // LDS index: one bit per 2^idx_shift bitmap words
+  int idx_shift = 0;                     // LDS index of the hash units: one bit per 2^idx_shift bitmap words
+  int hub_idx_shift = 0;                 // LDS index of the hub units: one bit per 16 << hub_idx_shift ids
   bool valid = false;
 };
 constexpr uint32_t kEdgeHashMax = 1u << 14;  // largest d_u an LDS hash table holds

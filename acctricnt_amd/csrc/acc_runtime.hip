@@ -60,7 +60,8 @@ void free_graph(acc_graph *g) {
 // bitmap within the batch.
 constexpr uint64_t kHubPoolBytes = 8ull << 30;  // HBM bitmaps of one hub batch
 constexpr uint32_t kHubBatch = 16384;           // most hubs of one batch (ACC_PARAM_BMP_BATCH)
-constexpr uint32_t kHubIdxBits = 1u << 17;      // LDS index bits of a hub unit (k_bmp_hub's kIdxWords)
+constexpr uint32_t kHashIdxBits = 1u << 17;     // LDS index bits of a hash unit (k_bmp_hash's kIdxWords)
+constexpr uint32_t kHubIdxBits = 1u << 18;      // LDS index bits of a hub unit (k_bmp_hub's kHubIdxWords)
 void build_edge_plan(uint32_t n, const uint32_t *off, const uint32_t *hs, uint32_t chunk, uint32_t hash_max,
                      uint32_t pool_hubs, std::vector<FwdUnit> (&units)[kEdgeClasses], std::vector<uint32_t> &hubs,
                      std::vector<uint32_t> &hub_batch, std::vector<uint32_t> &hub_unit_batch) {
@@ -153,7 +154,10 @@ int ensure_edge_plans(acc_graph *g) {
   if (p.valid) return ACC_OK;
   p.pool_words = std::max<uint64_t>(1, (static_cast<uint64_t>(g->n) + 31) / 32);
   p.idx_shift = 0;
-  while (((p.pool_words + (1ull << p.idx_shift) - 1) >> p.idx_shift) > kHubIdxBits) p.idx_shift++;
+  while (((p.pool_words + (1ull << p.idx_shift) - 1) >> p.idx_shift) > kHashIdxBits) p.idx_shift++;
+  const uint64_t groups = (static_cast<uint64_t>(g->n) + 15) / 16;  // the hub index's 16-id groups
+  p.hub_idx_shift = 0;
+  while (((groups + (1ull << p.hub_idx_shift) - 1) >> p.hub_idx_shift) > kHubIdxBits) p.hub_idx_shift++;
   const uint32_t pool_hubs = static_cast<uint32_t>(
       std::min<uint64_t>(g->bmp_batch, std::max<uint64_t>(1, kHubPoolBytes / (sizeof(uint32_t) * p.pool_words))));
   std::vector<FwdUnit> units[kEdgeClasses];

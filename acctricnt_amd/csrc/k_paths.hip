@@ -9,8 +9,8 @@
 //   behind an LDS index of N(u) (one bit per 32 ids) in a workgroup unit;
 //   larger sets spill to an n-bit bitmap in HBM,
 //   built for a batch of hubs by one kernel and shared by every workgroup
-//   that probes that hub, with an LDS index of one bit per bitmap word (per
-//   group of words beyond 2^17 words) checked before the HBM word, the
+//   that probes that hub, with an LDS index of one bit per 16 ids (per
+//   larger group beyond 2^18 groups) checked before the HBM word, the
 //   reference's BITMAP_SCALE_LOG index level (same file :320-382).  LDS hash
 //   tables of 2^12, 2^14 and 2^15 keys (up to 16384 ids; gfx950 gives a
 //   workgroup 160 KB of LDS) remain selectable for larger sets, though they
@@ -209,7 +209,11 @@ struct IndexedHash {
 };
 
 // N(u) of a hub as a bitmap in HBM, behind an LDS index with one bit per
-// 2^shift bitmap words (set when any of them holds a bit).
+// group of 16 << shift ids (set when N(u) holds one of them): finer than a
+// bitmap word, so that fewer probe elements go on to read an HBM word (for a
+// hub holding 2.5% of the ids, 33% of the elements pass a 16-id bit against
+// 55% a 32-id one).
+constexpr int kHubGroupLog = 4;
 struct HubBits {
   const uint32_t *idx;
   const uint32_t *__restrict__ bm;
@@ -219,7 +223,7 @@ struct HubBits {
     bool in[U];
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      const uint32_t iw = (static_cast<uint32_t>(a[q] < 0 ? 0 : a[q]) >> 5) >> shift;
+      const uint32_t iw = (static_cast<uint32_t>(a[q] < 0 ? 0 : a[q]) >> kHubGroupLog) >> shift;
       in[q] = a[q] >= 0 && ((idx[iw >> 5] >> (iw & 31)) & 1u);
     }
     uint32_t w[U];
@@ -406,26 +410,28 @@ __global__ __launch_bounds__(256) void k_hub_bits(const uint32_t *__restrict__ o
 }
 
 // Hub units of one batch: the unit's source is hub slot un.nlow of the pool.
+constexpr uint32_t kHubIdxWords = 8192;  // LDS index of 2^18 bits
 __global__ __launch_bounds__(kWG) void k_bmp_hub(const uint32_t *__restrict__ off, const int32_t *__restrict__ adj,
                                                  const uint32_t *__restrict__ rev, uint32_t n,
                                                  const FwdUnit *__restrict__ units, const uint32_t *__restrict__ pool,
                                                  uint64_t words, int shift, int32_t *cnt, int plus,
                                                  unsigned long long *sum) {
-  __shared__ uint32_t idx[kIdxWords];
+  __shared__ uint32_t idx[kHubIdxWords];
   __shared__ UnitLds L;
   const FwdUnit un = units[blockIdx.x];
   const int32_t u = un.b;
   const uint32_t su = off[u], eu = off[u + 1];
   const uint32_t nup = un.j1 - un.j0;
   const int tid = threadIdx.x;
-  const uint32_t iwords = static_cast<uint32_t>((((words + (1ull << shift) - 1) >> shift) + 31) / 32);
+  const uint64_t groups = (static_cast<uint64_t>(n) + 15) >> kHubGroupLog;
+  const uint32_t iwords = static_cast<uint32_t>((((groups + (1ull << shift) - 1) >> shift) + 31) / 32);
   for (uint32_t i = tid; i < iwords; i += kWG) idx[i] = 0;
   uint32_t cb = 0, ce = 0;
   if (static_cast<uint32_t>(tid) < nup) probe_of(off, adj, n, un.j0 + tid, cb, ce);
   L.ctr[tid] = 0;
   const uint32_t T = flatten(L, cb, ce - cb, nup);
   for (uint32_t i = su + tid; i < eu; i += kWG) {
-    const uint32_t iw = (static_cast<uint32_t>(adj[i]) >> 5) >> shift;
+    const uint32_t iw = (static_cast<uint32_t>(adj[i]) >> kHubGroupLog) >> shift;
     lds_or(&idx[iw >> 5], 1u << (iw & 31));
   }
   __syncthreads();
@@ -523,7 +529,7 @@ int launch_bitmap(acc_graph *g, int plus) {
                        p.pool_words, true);
     ACC_HIP_TRY(hipGetLastError());
     ACC_TRY(launch_units(k_bmp_hub, u1 - u0, st, g->d_off, g->d_adj, g->d_rev, g->n, p.d_units[kEdgeHub] + u0,
-                         p.d_pool, p.pool_words, p.idx_shift, g->d_cnt, plus, g->d_sum));
+                         p.d_pool, p.pool_words, p.hub_idx_shift, g->d_cnt, plus, g->d_sum));
     hipLaunchKernelGGL(k_hub_bits, dim3(h1 - h0), dim3(256), 0, st, g->d_off, g->d_adj, p.d_hubs + h0, p.d_pool,
                        p.pool_words, false);
     ACC_HIP_TRY(hipGetLastError());
