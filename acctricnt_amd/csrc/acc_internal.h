@@ -66,6 +66,8 @@ int bind_scratch(int device, unsigned long long **addr);
 // class) or 2^13 ids (the others).
 enum EdgeClass { kEdgeWave = 0, kEdgeHash12, kEdgeHash14, kEdgeHash15, kEdgeHub, kEdgeClasses };
 constexpr uint32_t kEdgeWaveDeg = 256;  // d_u of a wave unit
+constexpr int kEdgeWaveLog = 8;         // log2 kEdgeWaveDeg
+static_assert((1u << kEdgeWaveLog) == kEdgeWaveDeg, "wave-unit runs are searched in kEdgeWaveLog steps");
 constexpr uint32_t kEdgeWaveUp = 64;    // upper slots of a wave unit
 struct EdgePlan {
   FwdUnit *d_units[kEdgeClasses] = {};  // hub units: nlow = the hub's bitmap in its batch's pool

@@ -234,28 +234,30 @@ struct HubBits {
   }
 };
 
-// A window of sorted ids of N(u) in LDS; membership by a branch-free binary
-// search (top = the largest power of two <= len).
-struct SortedWin {
+// A window of 2^LOG sorted ids of N(u) in LDS, padded past the list's end
+// with INT_MAX (no valid id, above every id): membership by a branch-free
+// binary search of LOG steps with no bounds to check.
+template <int LOG>
+struct PaddedWin {
   const int32_t *s;
-  uint32_t len;
-  uint32_t top;
   // U searches advanced together, one LDS read of each per step.
   template <int U>
   __device__ __forceinline__ void has_n(const int32_t (&a)[U], bool (&hit)[U]) const {
     uint32_t pos[U];
 #pragma unroll
     for (int q = 0; q < U; q++) pos[q] = 0;
-    for (uint32_t step = top; step; step >>= 1) {
+#pragma unroll
+    for (uint32_t step = 1u << (LOG - 1); step; step >>= 1) {
       int32_t v[U];
 #pragma unroll
-      for (int q = 0; q < U; q++) v[q] = s[min(pos[q] + step, len) - 1];
+      for (int q = 0; q < U; q++) v[q] = s[pos[q] + step - 1];
 #pragma unroll
       for (int q = 0; q < U; q++)
-        if (pos[q] + step <= len && v[q] < a[q]) pos[q] += step;
+        if (v[q] < a[q]) pos[q] += step;
     }
+    // pos: the number of window ids below a, at most 2^LOG - 1
 #pragma unroll
-    for (int q = 0; q < U; q++) hit[q] = a[q] >= 0 && pos[q] < len && s[min(pos[q], len - 1)] == a[q];
+    for (int q = 0; q < U; q++) hit[q] = a[q] >= 0 && s[pos[q]] == a[q];
   }
 };
 
@@ -299,10 +301,9 @@ __global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ 
     int bits = 6;
     while ((1u << bits) < 2 * du) bits++;
     const uint32_t mask = (1u << bits) - 1;
-    uint32_t top = 1;
-    while (top * 2 <= du) top *= 2;
     if (MERGE) {
-      for (uint32_t i = lane; i < du; i += 64) t[i] = adj[su + i];
+      // the sorted run padded to kEdgeWaveDeg ids (du <= kEdgeWaveDeg)
+      for (uint32_t i = lane; i < kEdgeWaveDeg; i += 64) t[i] = i < du ? adj[su + i] : INT32_MAX;
     } else {
       for (uint32_t i = lane; i <= mask; i += 64) t[i] = -1;
       wave_lds_sync();
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kWG) void k_edge_wave(const uint32_t *__restrict__ 
       }
       bool hit[kU];
       if (MERGE)
-        SortedWin{t, du, top}.has_n<kU>(a, hit);
+        PaddedWin<kEdgeWaveLog>{t}.has_n<kU>(a, hit);
       else
         LdsHash{t, bits, mask}.has_n<kU>(a, hit);
 #pragma unroll
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(kWG) void k_merge_win(const uint32_t *__restrict__ 
   const uint32_t nwin = (du + W - 1) / W;
   for (uint32_t w = 0; w < nwin; w++) {
     const uint32_t w0 = w * W, wl = du - w0 < W ? du - w0 : W;
-    for (uint32_t i = tid; i < wl; i += kWG) win[i] = adj[su + w0 + i];
+    for (uint32_t i = tid; i < W; i += kWG) win[i] = i < wl ? adj[su + w0 + i] : INT32_MAX;
     // slice end: the first element of N(v) at or above the next window's first id
     uint32_t end = ce;
     if (w + 1 < nwin && cur < ce) {
@@ -482,9 +483,7 @@ __global__ __launch_bounds__(kWG) void k_merge_win(const uint32_t *__restrict__ 
     }
     const uint32_t T = flatten(L, cur, end - cur, nup);
     __syncthreads();
-    uint32_t top = 1;
-    while (top * 2 <= wl) top *= 2;
-    probe_flat<U>(adj, SortedWin{win, wl, top}, L, T);
+    probe_flat<U>(adj, PaddedWin<WBITS>{win}, L, T);
     cur = end;
     __syncthreads();
   }

@@ -5,7 +5,8 @@ here with g++, and an emulation of the wave-pack kernel's index arithmetic
 the windowed probe searches of acctricnt_amd/csrc/flat_window.h that the
 forward and per-edge kernels share (lds_search over a 512-entry prefix in
 LDS, lane_window over the lanes of a wave) and of its per-probe hit adds
-(add_row_hits).
+(add_row_hits), and the merge path's padded window search
+(PaddedWin in acctricnt_amd/csrc/k_paths.hip).
 
 For every pack the emulation recomputes what each lane of the kernel
 computes -- the sources' upper-slot, lower-entry and hash-table prefixes, each
@@ -432,3 +433,45 @@ def test_row_hits_by_segment(seed):
                 want[int(k[l])] = want.get(int(k[l]), 0) + 1
         assert adds == want
         assert n == len(want)
+
+
+INT32_MAX = 2**31 - 1
+
+
+def padded_win_has(window, log, a, first_read=lambda pos, step: pos + step - 1):
+    """PaddedWin<log>::has_n for one element a: the window (sorted, len <=
+    2^log) padded with INT32_MAX, log branch-free steps, then one compare."""
+    s = list(window) + [INT32_MAX] * ((1 << log) - len(window))
+    pos, step = 0, 1 << (log - 1)
+    while step:
+        i = first_read(pos, step)
+        assert 0 <= i < (1 << log), "a read outside the padded window"
+        if s[i] < a:
+            pos += step
+        step >>= 1
+    assert 0 <= pos < (1 << log)
+    return a >= 0 and s[pos] == a
+
+
+@pytest.mark.parametrize("log", [3, 8, 11])
+def test_padded_window_search(log):
+    """Membership equals the set's for full, partial and empty windows, for
+    ids below, inside, between and above the window's (the last window of a
+    list is partial and every window's slice may reach past its last id)."""
+    rng = np.random.default_rng(300 + log)
+    for fill in (0, 1, 3, (1 << log) - 1, 1 << log):
+        for _ in range(20):
+            w = np.sort(rng.choice(5 * (1 << log) + 10, size=fill, replace=False)).astype(np.int64)
+            ws = set(int(x) for x in w)
+            for a in list(range(-1, 5 * (1 << log) + 12, max(1, (1 << log) // 64))) + [int(x) for x in w]:
+                assert padded_win_has(w, log, a) == (a >= 0 and a in ws), (log, fill, a)
+
+
+def test_padded_window_search_catches_an_off_by_one():
+    w = list(range(0, 40, 2))
+    # reading one entry too far at every step (kept inside the window) misses ids
+    bad = sum(padded_win_has(w, 5, a, first_read=lambda pos, step: min(pos + step, 31)) != (a in w)
+              for a in range(40))
+    assert bad > 0
+    src = open(os.path.join(CSRC, "k_paths.hip")).read()
+    assert "v[q] = s[pos[q] + step - 1];" in src and "INT32_MAX" in src
