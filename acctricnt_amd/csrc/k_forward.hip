@@ -636,7 +636,11 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
 // each, the following tasks kWaves wave packs each; workgroup w runs tasks
 // w, w + gridDim.x, ... (one task per workgroup when the grid covers them, a
 // persistent grid otherwise).  The task index is workgroup-uniform, so the
-// barrier that separates consecutive tasks' use of LDS is reached by all.
+// barrier that separates consecutive workgroup units' use of LDS is reached
+// by all.  Wave packs use only their wave's part of LDS, so once its
+// workgroup units are done each wave goes through its packs on its own, with
+// no workgroup barrier: a wave does not wait for the slowest pack of its
+// workgroup before taking its next one.
 // kSlots false: the triangle total only, accumulated into the partial sums at total.
 template <int U, bool kSlots>
 __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ off,
@@ -649,14 +653,14 @@ __global__ __launch_bounds__(kWG) void k_fwd_count(const uint32_t *__restrict__ 
   __shared__ __align__(16) unsigned char smem[kCountLdsBytes];
   const uint32_t tasks = n_block + (n_wave + kWaves - 1) / kWaves;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (uint32_t t = blockIdx.x; t < tasks; t += gridDim.x) {
-    if (t < n_block) {
-      fwd_block_unit<U, kSlots>(off, adj, rev, block_units[t], cnt, total, smem);
-    } else {
-      const uint32_t ui = (t - n_block) * kWaves + wave;
-      if (ui < n_wave) fwd_pack_unit<U, kSlots>(off, hs, adj, rev, wave_units[ui], cnt, total, smem, wave);
-    }
+  uint32_t t = blockIdx.x;
+  for (; t < n_block; t += gridDim.x) {
+    fwd_block_unit<U, kSlots>(off, adj, rev, block_units[t], cnt, total, smem);
     __syncthreads();
+  }
+  for (; t < tasks; t += gridDim.x) {
+    const uint32_t ui = (t - n_block) * kWaves + wave;
+    if (ui < n_wave) fwd_pack_unit<U, kSlots>(off, hs, adj, rev, wave_units[ui], cnt, total, smem, wave);
   }
 }
 
