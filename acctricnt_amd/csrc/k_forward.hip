@@ -180,18 +180,6 @@ __device__ __forceinline__ int table_bits_for(uint32_t nl) {
   return bits;
 }
 
-// Open-addressing set of N<(b) in LDS: key = vertex id, value = its position in N<(b).
-__device__ __forceinline__ int lds_find(const int32_t *keys, const uint16_t *kpos, int bits, uint32_t mask,
-                                        int32_t a) {
-  uint32_t h = hash_slot(a, bits);
-  for (;;) {
-    const int32_t k = keys[h];
-    if (k == a) return kpos[h];
-    if (k == -1) return -1;
-    h = (h + 1) & mask;
-  }
-}
-
 // Inserts N<(b) = adj[sb, sb+nl) with a stride of `step` threads starting at `t`.
 __device__ __forceinline__ void lds_insert(int32_t *keys, uint16_t *kpos, int bits, uint32_t mask,
                                            const int32_t *__restrict__ adj, uint32_t sb, uint32_t nl, int t,
@@ -204,30 +192,84 @@ __device__ __forceinline__ void lds_insert(int32_t *keys, uint16_t *kpos, int bi
   }
 }
 
-// Membership of N<(b) as an open-addressing hash table in LDS.
+// Membership of N<(b) as an open-addressing hash table in LDS: key = vertex
+// id, value (kpos) = its position in N<(b).  A round's U elements (a[q] < 0:
+// no element) are looked up together: the first probe of every row is read
+// before any row is decided, so the LDS reads overlap, and only a row whose
+// first probe meets another key walks on.
 struct HashSet {
   const int32_t *keys;
   const uint16_t *kpos;
   int bits;
   uint32_t mask;
-  __device__ __forceinline__ int find(int32_t a) const { return lds_find(keys, kpos, bits, mask, a); }
+  template <int U>
+  __device__ __forceinline__ void walk_rows(const int32_t (&a)[U], uint32_t (&h)[U], bool (&hit)[U]) const {
+    int32_t k[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      h[q] = hash_slot(a[q], bits);
+      k[q] = keys[h[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      bool found = a[q] >= 0 && k[q] == a[q];
+      bool more = a[q] >= 0 && k[q] != a[q] && k[q] != -1;
+      while (more) {
+        h[q] = (h[q] + 1) & mask;
+        const int32_t kk = keys[h[q]];
+        found = kk == a[q];
+        more = !found && kk != -1;
+      }
+      hit[q] = found;
+    }
+  }
+  template <int U>
+  __device__ __forceinline__ void contains_rows(const int32_t (&a)[U], bool (&hit)[U]) const {
+    uint32_t h[U];
+    walk_rows(a, h, hit);
+  }
+  // the position of one element a in N<(b), -1 when absent
+  __device__ __forceinline__ int find(int32_t a) const {
+    const int32_t a1[1] = {a};
+    uint32_t h[1];
+    bool hit[1];
+    walk_rows(a1, h, hit);
+    return hit[0] ? static_cast<int>(kpos[h[0]]) : -1;
+  }
 };
 
 // Membership of N<(b) ⊂ [lo, b), lo = min N(b), as an exact LDS bitmap over
 // [lo, b) with the position of a in N<(b) recovered from the index of the
-// first element of a's 32-bit word (pre) plus a popcount: one LDS read
-// decides a probe.  Callers only ask for a < b.
+// first element of a's 32-bit word (pre) plus a popcount.  A round's U
+// elements (a[q] < 0: no element, and then r < 0 since lo >= 0) are looked up
+// together with no branch, so the LDS reads overlap; a row without an element
+// reads word 0.  Callers only ask for a < b.
 struct BitSet {
   const uint32_t *bm;
   const uint16_t *pre;
   int32_t lo;
+  template <int U>
+  __device__ __forceinline__ void contains_rows(const int32_t (&a)[U], bool (&hit)[U]) const {
+    uint32_t w[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const int32_t r = a[q] - lo;
+      w[q] = bm[r >= 0 ? (r >> 5) : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const int32_t r = a[q] - lo;
+      hit[q] = r >= 0 && ((w[q] >> (r & 31)) & 1u);
+    }
+  }
+  // the position of one element a in N<(b), -1 when absent (both words
+  // read with no branch)
   __device__ __forceinline__ int find(int32_t a) const {
     const int32_t r = a - lo;
-    if (r < 0) return -1;
-    const uint32_t w = bm[r >> 5];
+    const int32_t wi = r >= 0 ? (r >> 5) : 0;
+    const uint32_t w = bm[wi], p = pre[wi];
     const uint32_t bit = 1u << (r & 31);
-    if (!(w & bit)) return -1;
-    return pre[r >> 5] + __popc(w & (bit - 1));
+    return r >= 0 && (w & bit) ? static_cast<int>(p + __popc(w & (bit - 1))) : -1;
   }
 };
 
@@ -321,16 +363,13 @@ __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__
         idx[q] = s_cb[kq[q]] + e;
         a[q] = e < T ? adj[idx[q]] : -1;
       }
+      // one row at a time: looking the U rows up together, as the
+      // triangle-only pass does, made the persistent grid 1.7x slower here
 #pragma unroll
       for (int q = 0; q < U; q++) {
-        bool hit = false;
-        if (a[q] >= 0) {
-          const int pos = set.find(a[q]);
-          if (pos >= 0) {
-            count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
-            hit = true;
-          }
-        }
+        const int pos = set.find(a[q]);
+        const bool hit = pos >= 0;
+        if (hit) count_triangle(cnt, ctr, kLdsCap, sb, idx[q], pos);
         count_round(s_reg, kq[q], hit, lane);
       }
     }
@@ -363,8 +402,10 @@ __device__ __forceinline__ uint32_t block_probe_flat(const int32_t *__restrict__
       const uint32_t e1 = e0 + kStride;
       const bool more = e1 < T;  // wave-uniform
       fetch(more ? e1 : e0, nxt);
+      bool hit[U];
+      set.contains_rows(cur.v, hit);
 #pragma unroll
-      for (int q = 0; q < U; q++) found += cur.v[q] >= 0 && set.find(cur.v[q]) >= 0;
+      for (int q = 0; q < U; q++) found += hit[q];
       if (!more) break;
       e0 = e1;
       cur = nxt;
@@ -590,27 +631,41 @@ __device__ __forceinline__ void fwd_pack_unit(const uint32_t *__restrict__ off, 
       idx[q] = static_cast<uint32_t>(__shfl(cbP, kq[q])) + eq[q];
       a[q] = eq[q] < T ? adj[idx[q]] : -1;
     }
+    // the first probe of every row's table is read before any row is
+    // decided, so the LDS reads overlap; a row without an element, or whose
+    // source has no table, reads entry 0 of the wave's tables and misses
+    uint32_t pm[U], h[U];
+    int32_t k[U];
 #pragma unroll
     for (int q = 0; q < U; q++) {
-      const uint32_t pm = static_cast<uint32_t>(__shfl(meta, kq[q]));
-      const uint32_t tb = pm & 511u;
-      const int tbits = static_cast<int>((pm >> 9) & 15u);
-      const uint32_t nlb = pm >> 13;
-      bool hit = false;
-      if (a[q] >= 0 && tbits > 0) {
-        const int pos = lds_find(keys + tb, kpos + tb, tbits, (1u << tbits) - 1, a[q]);
-        if (pos >= 0) {
-          if (kSlots) {
-            atomicAdd(&cnt[idx[q]], 1);
-            atomicAdd(&ctr[nlb + pos], 1);
-          }
-          hit = true;
-        }
+      pm[q] = static_cast<uint32_t>(__shfl(meta, kq[q]));
+      const int tbits = static_cast<int>((pm[q] >> 9) & 15u);
+      const bool ok = a[q] >= 0 && tbits > 0;
+      h[q] = ok ? hash_slot(a[q], tbits) : 0;
+      k[q] = keys[(pm[q] & 511u) + h[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint32_t tb = pm[q] & 511u;
+      const int tbits = static_cast<int>((pm[q] >> 9) & 15u);
+      const bool ok = a[q] >= 0 && tbits > 0;
+      bool hit = ok && k[q] == a[q];
+      bool more = ok && k[q] != a[q] && k[q] != -1;
+      while (more) {
+        h[q] = (h[q] + 1) & ((1u << tbits) - 1);
+        const int32_t kk = keys[tb + h[q]];
+        hit = kk == a[q];
+        more = !hit && kk != -1;
       }
-      if (kSlots)
+      if constexpr (kSlots) {
+        if (hit) {
+          atomicAdd(&cnt[idx[q]], 1);
+          atomicAdd(&ctr[(pm[q] >> 13) + kpos[tb + h[q]]], 1);
+        }
         count_round(reg, kq[q], hit, lane);
-      else
+      } else {
         found += hit;
+      }
     }
   }
   if (!kSlots) {
