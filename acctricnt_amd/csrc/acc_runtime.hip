@@ -735,7 +735,7 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap) {
       {ACC_PARAM_WAVE_SLOTS, "wave_slots", quick ? std::vector<int64_t>{16, 64} : std::vector<int64_t>{0, 16, 32, 64}},
       {ACC_PARAM_WAVE_LOW, "wave_low", quick ? std::vector<int64_t>{256} : std::vector<int64_t>{64, 128, 256}},
       {ACC_PARAM_CHUNK, "chunk", quick ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{128, 256, 512}},
-      {ACC_PARAM_PERSIST, "persist", quick ? std::vector<int64_t>{0, 4} : std::vector<int64_t>{0, 2, 4, 8}},
+      {ACC_PARAM_PERSIST, "persist", quick ? std::vector<int64_t>{0, 8, 24} : std::vector<int64_t>{0, 4, 8, 16, 24, 32, 64}},
   };
   std::string trials;
   auto add_trial = [&](const std::string &path, const char *param, int64_t value, float ms) {
