@@ -124,7 +124,8 @@ struct acc_graph {
   uint32_t wave_slots = accfwd::kWaveSlots;  // wave-unit bin: at most this many upper slots
   uint32_t wave_low = accfwd::kWaveCap;      // ... and |N<(b)| at most this
   uint32_t chunk = accfwd::kChunk;           // upper slots per workgroup unit
-  uint32_t persist = 0;                      // counting grid: 0 one workgroup per task, else per-CU workgroups
+  uint32_t persist = 256;                    // counting grid: 0 one workgroup per task, else at most this many
+                                             // workgroups per CU looping over the tasks
   uint32_t bmp_hash_max = 256;               // bitmap path: d_u above this goes to the HBM bitmaps
   uint32_t bmp_batch = 4096;                 // bitmap path: hub bitmaps built and probed per batch
   uint32_t shard_lo = 0, shard_hi = 0;       // forward sources [shard_lo, shard_hi) (the whole graph: 0, n)

@@ -657,7 +657,7 @@ int acc_graph_set_param(acc_graph *g, int param, int64_t value) {
       g->h_work.clear();  // the shard work estimate counts units of `chunk` slots
       return upload_plans(g);
     case ACC_PARAM_PERSIST:
-      if (value < 0 || value > 64) return acc_set_error(ACC_EINVAL, "persist must lie in [0, 64]");
+      if (value < 0 || value > 256) return acc_set_error(ACC_EINVAL, "persist must lie in [0, 256]");
       g->persist = static_cast<uint32_t>(value);
       return ACC_OK;
     case ACC_PARAM_AUTO_PATH:
@@ -735,7 +735,7 @@ int acc_tune(acc_graph *g, uint32_t flags, char *report, int report_cap) {
       {ACC_PARAM_WAVE_SLOTS, "wave_slots", quick ? std::vector<int64_t>{16, 64} : std::vector<int64_t>{0, 16, 32, 64}},
       {ACC_PARAM_WAVE_LOW, "wave_low", quick ? std::vector<int64_t>{256} : std::vector<int64_t>{64, 128, 256}},
       {ACC_PARAM_CHUNK, "chunk", quick ? std::vector<int64_t>{256, 512} : std::vector<int64_t>{128, 256, 512}},
-      {ACC_PARAM_PERSIST, "persist", quick ? std::vector<int64_t>{0, 8, 24} : std::vector<int64_t>{0, 4, 8, 16, 24, 32, 64}},
+      {ACC_PARAM_PERSIST, "persist", quick ? std::vector<int64_t>{0, 24, 256} : std::vector<int64_t>{0, 8, 24, 64, 128, 256}},
   };
   std::string trials;
   auto add_trial = [&](const std::string &path, const char *param, int64_t value, float ms) {

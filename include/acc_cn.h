@@ -112,7 +112,8 @@ int acc_cn_timings(acc_graph *g, float *ms, int cap, char *names, int names_cap)
 #define ACC_PARAM_WAVE_LOW 3   /* ... and |N<(b)| at most this (0..256) run one source per wave */
 #define ACC_PARAM_CHUNK 4      /* upper slots per workgroup unit (64..512), all paths */
 #define ACC_PARAM_AUTO_PATH 5  /* the path ACC_PATH_AUTO runs */
-#define ACC_PARAM_PERSIST 6    /* forward counting grid: 0 one workgroup per task, k>0 at most k per CU (persistent) */
+#define ACC_PARAM_PERSIST 6    /* forward counting grid: 0 one workgroup per task, k in [1, 256] at most k per CU
+                                  looping over the tasks (persistent; default 256) */
 #define ACC_PARAM_BMP_HASH_MAX 7 /* bitmap path: sources with d_u at most this (0..16384, default 256) keep
                                     N(u) in an LDS hash table, larger ones in an HBM bitmap; both behind an
                                     LDS index of N(u) */

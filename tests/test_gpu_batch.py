@@ -124,7 +124,7 @@ def test_params_rejected(ctx):
     g = G.gen_rmat(10, 8, seed=1)
     with ctx.upload(g) as rg:
         for name, value in (("fwd_unroll", 3), ("wave_slots", 65), ("wave_low", 257), ("chunk", 63),
-                            ("chunk", 513), ("persist", 65), ("auto_path", 0)):
+                            ("chunk", 513), ("persist", 257), ("auto_path", 0)):
             with pytest.raises(_ffi.AccError):
                 rg.set_param(name, value)
         with pytest.raises(_ffi.AccError):
