@@ -101,7 +101,7 @@ def test_capture_replay_bitmap_hubs(ctx):
             rg.count_all_edges("forward")
 
 
-PARAM_CASES = [("persist", 1), ("persist", 4), ("persist", 24), ("persist", 64), ("wave_slots", 0), ("wave_slots", 16),
+PARAM_CASES = [("persist", 0), ("persist", 1), ("persist", 4), ("persist", 24), ("persist", 64), ("wave_slots", 0), ("wave_slots", 16),
                ("wave_low", 0), ("wave_low", 64), ("chunk", 64), ("chunk", 128), ("chunk", 300)]
 
 
