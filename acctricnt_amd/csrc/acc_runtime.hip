@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <numeric>
 #include <thread>
 #include <new>
 
@@ -140,6 +141,20 @@ int upload_plans(acc_graph *g) {
   const uint32_t device_waves = static_cast<uint32_t>(std::max(1, g->ctx->num_cus)) * 32u;
   build_forward_plan(g->shard_lo, g->shard_hi, g->h_off.data(), g->h_hs.data(), g->h_lo.data(), g->wave_slots,
                      g->wave_low, g->chunk, device_waves, wave, block, big);
+  // Workgroup units in a stride order (the golden-ratio stride, coprime with
+  // their number, so a permutation): the tasks that run at the same time
+  // then come from distant sources.  At R-MAT s22 this took the
+  // triangle-only pass from 6.7 to 6.1-6.4 ms and the per-slot pass 2%
+  // faster; the wave packs keep source order (permuting them as well slowed
+  // the 512-graph batch by 9%).  Counts do not depend on the order.
+  if (block.size() > 1) {
+    const size_t n = block.size();
+    size_t S = static_cast<size_t>(static_cast<double>(n) * 0.6180339887) | 1u;
+    while (std::gcd(S, n) != 1) S += 2;
+    std::vector<FwdUnit> p(n);
+    for (size_t j = 0; j < n; j++) p[j] = block[(j * S) % n];
+    block.swap(p);
+  }
   PlanList lists[] = {{&wave, &g->d_fwd_wave, &g->n_fwd_wave},
                       {&block, &g->d_fwd_block, &g->n_fwd_block},
                       {&big, &g->d_fwd_big, &g->n_fwd_big}};
